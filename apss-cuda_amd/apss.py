@@ -1,0 +1,203 @@
+"""ctypes binding of libapss.so (C-ABI declared in include/apss.h) and of the
+synthetic CSR generator in libapss_host.so.
+
+This module is the Python-side mirror of the C entry points; it holds no
+compute of its own.  If libapss.so is missing it raises at load time -- there
+is no fallback path.
+
+Workload definitions (WORKLOADS) follow the five configurations the reference
+method is benchmarked on: row lengths a truncated power law, Zipf-ranked
+feature ids (id 0 most frequent), uniform(0,1] or lognormal(0,1) values,
+L2-normalised rows.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapss.so")
+HOST_LIB_PATH = os.path.join(HERE, "libapss_host.so")
+
+APSS_OK, APSS_E_INVALID, APSS_E_NOMEM, APSS_E_DEVICE, APSS_E_UNSUPPORTED, APSS_E_CANCELLED, APSS_E_BUSY = (
+    0, -1, -2, -3, -4, -5, -6)
+PREC_FP32, PREC_BF16_HEAD = 0, 1
+
+PAIR_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("score", "<f4")])
+
+
+class CSR(ctypes.Structure):
+    _fields_ = [("n_rows", c_int64), ("n_cols", c_int64), ("nnz", c_int64),
+                ("row_ptr", c_void_p), ("col", c_void_p), ("val", c_void_p),
+                ("on_device", c_int32), ("reserved_", c_int32)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("device", c_int32), ("precision", c_int32), ("sort_output", c_int32),
+                ("rescore_all", c_int32), ("output_on_device", c_int32),
+                ("dense_features", c_int32), ("mem_budget", c_int64), ("stream", c_void_p),
+                ("reserved_", c_int64 * 4)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("count", c_int64), ("pairs", c_void_p), ("on_device", c_int32),
+                ("p_used", c_int32), ("cells", c_int64), ("band", c_int64),
+                ("products", c_int64), ("head_pairs", c_int64),
+                ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
+                ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
+                ("internal_", c_void_p)]
+
+
+class GenSpec(ctypes.Structure):
+    _fields_ = [("n", c_int64), ("row_offset", c_int64), ("d", c_int32), ("kmin", c_int32),
+                ("kmax", c_int32), ("alpha", c_double), ("zipf_s", c_double),
+                ("values", c_int32), ("pad_", c_int32), ("seed", c_uint64)]
+
+
+# Every symbol declared in include/apss.h.
+ABI_SYMBOLS = [
+    "apss_version", "apss_strerror", "apss_last_error", "apss_opts_init", "apss_run",
+    "apss_run_cb", "apss_run_batched", "apss_result_free", "apss_index_create",
+    "apss_index_insert", "apss_index_query", "apss_index_rows", "apss_index_destroy",
+    "apss_server_create", "apss_server_submit", "apss_server_wait", "apss_server_cancel",
+    "apss_server_destroy", "similarity_join_device", "apss_compat_last_count",
+]
+
+_lib = None
+_host = None
+
+
+def lib():
+    """Load libapss.so (raises OSError if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.apss_version.restype = c_char_p
+        L.apss_strerror.restype = c_char_p
+        L.apss_strerror.argtypes = [c_int32]
+        L.apss_last_error.restype = c_char_p
+        L.apss_opts_init.argtypes = [POINTER(Opts)]
+        L.apss_run.argtypes = [POINTER(CSR), c_double, c_int32, POINTER(Opts), POINTER(Result)]
+        L.apss_run.restype = c_int32
+        L.apss_result_free.argtypes = [POINTER(Result)]
+        _lib = L
+    return _lib
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise OSError(f"{HOST_LIB_PATH} not built")
+        H = ctypes.CDLL(HOST_LIB_PATH)
+        H.apss_gen_alpha.restype = c_double
+        H.apss_gen_alpha.argtypes = [c_int32, c_int32, c_double]
+        H.apss_gen_rowptr.restype = c_int64
+        H.apss_gen_rowptr.argtypes = [POINTER(GenSpec), c_void_p]
+        H.apss_gen_fill.restype = c_int32
+        H.apss_gen_fill.argtypes = [POINTER(GenSpec), c_void_p, c_void_p, c_void_p]
+        _host = H
+    return _host
+
+
+class ApssError(RuntimeError):
+    def __init__(self, code, detail=""):
+        self.code = code
+        msg = lib().apss_strerror(code).decode()
+        super().__init__(f"apss error {code}: {msg}" + (f" ({detail})" if detail else ""))
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads
+
+SEED_BASE = 0x41505353
+
+WORKLOADS = {
+    # name: n, d, mean nnz/row, kmin, kmax, zipf s, values, thresholds
+    "small": dict(n=20_000, d=50_000, mean=40, kmin=5, kmax=500, zipf=1.1, values="uniform", t=(0.7,), index=0),
+    "large": dict(n=1_000_000, d=1_000_000, mean=100, kmin=5, kmax=5000, zipf=1.05, values="lognormal",
+                  t=(0.5, 0.7, 0.9), index=1),
+    "batched": dict(n=None, d=100_000, mean=None, kmin=5, kmax=5000, zipf=1.1, values="uniform", t=(0.8,), index=2),
+    "mixed": dict(n=500_000, d=500_000, mean=80, kmin=5, kmax=5000, zipf=1.2, values="uniform", t=(0.5,), index=3),
+    "serving": dict(n=2_000_000, d=1_000_000, mean=60, kmin=5, kmax=5000, zipf=1.1, values="lognormal",
+                    t=(0.6, 0.8), index=4),
+}
+
+
+def generate(n, d, mean, kmin=5, kmax=5000, zipf=1.1, values="uniform", seed=SEED_BASE, row_offset=0,
+             alpha=None):
+    """Deterministic CSR (row_ptr int64, col int32, val float32) from libapss_host.so."""
+    H = host_lib()
+    kmax = min(kmax, d)
+    kmin = min(kmin, kmax)
+    if alpha is None:
+        alpha = H.apss_gen_alpha(kmin, kmax, float(mean))
+    spec = GenSpec(n=n, row_offset=row_offset, d=d, kmin=kmin, kmax=kmax, alpha=alpha, zipf_s=zipf,
+                   values=1 if values == "lognormal" else 0, pad_=0, seed=seed)
+    rp = np.empty(n + 1, dtype=np.int64)
+    nnz = H.apss_gen_rowptr(ctypes.byref(spec), rp.ctypes.data)
+    if nnz < 0:
+        raise ValueError("bad generator spec")
+    col = np.empty(max(nnz, 1), dtype=np.int32)[:nnz]
+    val = np.empty(max(nnz, 1), dtype=np.float32)[:nnz]
+    if H.apss_gen_fill(ctypes.byref(spec), rp.ctypes.data, col.ctypes.data, val.ctypes.data) != 0:
+        raise MemoryError("generator failed")
+    return rp, col, val
+
+
+def workload(name, n=None, seed_offset=0):
+    """Matrix of a named workload, optionally with fewer rows (same distribution)."""
+    w = WORKLOADS[name]
+    n = n if n is not None else w["n"]
+    return generate(n, w["d"], w["mean"], w["kmin"], w["kmax"], w["zipf"], w["values"],
+                    seed=SEED_BASE + w["index"] + seed_offset) + (w["d"],)
+
+
+# ---------------------------------------------------------------------------
+# calls
+
+def make_csr(rp, col, val, d, on_device=False):
+    return CSR(n_rows=len(rp) - 1, n_cols=d, nnz=len(col), row_ptr=rp.ctypes.data,
+               col=col.ctypes.data if len(col) else None, val=val.ctypes.data if len(val) else None,
+               on_device=1 if on_device else 0)
+
+
+def make_opts(**kw):
+    o = Opts()
+    lib().apss_opts_init(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def result_pairs(res):
+    """Copy a host-resident result into a numpy structured array."""
+    n = res.count
+    if n == 0 or not res.pairs:
+        return np.zeros(0, dtype=PAIR_DTYPE)
+    buf = (ctypes.c_char * (n * 12)).from_address(res.pairs)
+    return np.frombuffer(buf, dtype=PAIR_DTYPE, count=n).copy()
+
+
+def run(rp, col, val, d, t, p=0, **opts):
+    """Self-join APSS through apss_run; returns (pairs, info)."""
+    L = lib()
+    X = make_csr(rp, col, val, d)
+    o = make_opts(**opts)
+    r = Result()
+    rc = L.apss_run(ctypes.byref(X), float(t), int(p), ctypes.byref(o), ctypes.byref(r))
+    if rc != APSS_OK:
+        raise ApssError(rc, L.apss_last_error().decode())
+    try:
+        pairs = result_pairs(r) if not r.on_device else None
+        info = {f: getattr(r, f) for f, _ in Result._fields_ if f not in ("pairs", "internal_")}
+    finally:
+        L.apss_result_free(ctypes.byref(r))
+    return pairs, info
+
+
+def sort_pairs(pairs):
+    return pairs[np.lexsort((pairs["j"], pairs["i"]))]

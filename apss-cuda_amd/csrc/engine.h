@@ -1,0 +1,145 @@
+// Host-side engine of libapss.so: device memory, index build, scan launches.
+#pragma once
+#include <memory>
+
+#include "apss_internal.h"
+
+namespace apss {
+
+// ---- kernels (defined in index_build.hip / tile_scan.hip / head_mfma.hip) ----
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v);
+__global__ void k_copy_i64(int64_t* dst, const int64_t* src, int64_t n);
+__global__ void k_validate_norms(int64_t n, int64_t d, int64_t nnz, const int64_t* rp,
+                                 const int32_t* col, const float* val, int32_t p, float* norm,
+                                 float* hnorm, uint32_t* err);
+__global__ void k_col_hist(int64_t nnz, const int32_t* col, int32_t p, int64_t chunk, int32_t* cnt);
+__global__ void k_cursor_scatter(int64_t n, int64_t row_off, const int64_t* rp, const int32_t* col,
+                                 const float* val, const int32_t* fmap, unsigned long long* cur,
+                                 int32_t* srow, float* sval);
+__global__ void k_sort_small(const int32_t* lists, int32_t nl, const int64_t* sptr, int32_t* srow,
+                             float* sval);
+__global__ void k_sort_large(const int32_t* lists, const int64_t* sptr, int32_t* srow, float* sval);
+__global__ void k_tile_count(BuildSide s);
+__global__ void k_scan_i64(int64_t* a, int64_t n);
+__global__ void k_tile_ptr_scan(int32_t* ptr, uint64_t* bmp, int32_t KD);
+__global__ void k_tile_scatter(BuildSide s);
+__global__ void k_band_scan(ScanArgs a);
+__global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
+                          const int32_t* colI, const float* valI, const int64_t* rpJ,
+                          const int32_t* colJ, const float* valJ, double t, int64_t i_off,
+                          int64_t j_off, apss_pair* out, unsigned long long* out_cnt,
+                          int64_t out_cap);
+
+// ---- device memory --------------------------------------------------------
+
+// Owns every device allocation of one call (or one index); enforces a byte
+// budget and frees everything on destruction.
+class Arena {
+ public:
+  Arena(int device, int64_t budget) : device_(device), budget_(budget) {}
+  ~Arena();
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  template <class T>
+  T* alloc(int64_t count) {
+    return static_cast<T*>(raw(count < 1 ? sizeof(T) : (size_t)count * sizeof(T)));
+  }
+  void release(void* p);  // free one allocation early
+  void* detach(void* p);  // hand one allocation to the caller (not freed here)
+  int64_t used() const { return used_; }
+  int64_t peak() const { return peak_; }
+
+ private:
+  void* raw(size_t bytes);
+  int device_;
+  int64_t budget_;
+  int64_t used_ = 0, peak_ = 0;
+  std::vector<std::pair<void*, size_t>> blocks_;
+};
+
+// ---- index pieces -----------------------------------------------------------
+
+// Feature classes and the cursor-tier lists of a corpus matrix.
+struct Tiers {
+  int64_t d = 0;
+  int32_t p = 0;
+  int32_t KD = 64, KW = 1;
+  int32_t n_bitmap = 0;          // features in the bitmap tier
+  int64_t n_cursor_entries = 0;
+  int32_t* fmap = nullptr;       // d
+  int64_t* sptr = nullptr;       // d+1
+  int32_t* srow = nullptr;
+  float* sval = nullptr;
+};
+
+// One matrix side: CSR on the device, norms and the per-tile structures.
+struct Side {
+  int64_t n = 0, nnz = 0, d = 0;
+  const int64_t* rp = nullptr;
+  const int32_t* col = nullptr;
+  const float* val = nullptr;
+  float* norm = nullptr;
+  float* hnorm = nullptr;
+  int32_t maxlen = 0;
+  float maxnorm = 0.f;
+  int32_t T = 0;
+  int32_t* ptr = nullptr;
+  uint64_t* bmp = nullptr;
+  int64_t* dbase = nullptr;
+  int64_t* sbase = nullptr;
+  uint8_t* drow = nullptr;
+  float* dval = nullptr;
+  int4* srec = nullptr;
+  int64_t nD = 0, nS = 0;
+  TileSide view() const;
+};
+
+struct Timer {  // hipEvent pair on a stream
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit Timer(hipStream_t st);
+  ~Timer();
+  void start();
+  double stop_ms();  // synchronises
+};
+
+struct RunConfig {
+  double t = 0.5;
+  int32_t p = 0;
+  int32_t self = 1;
+  int32_t rescore_all = 0;
+  int32_t kd_cap = 0;
+  int64_t i_off = 0, j_off = 0;
+};
+
+struct ScanOutput {
+  apss_pair* pairs = nullptr;   // device, owned by the arena
+  int64_t count = 0;
+  int64_t band = 0, products = 0, head_pairs = 0, cells = 0;
+  double ms_scan = 0, ms_rescore = 0;
+};
+
+// Upload (if needed) a CSR matrix and validate it; computes norms.
+Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p);
+// Histogram + tier selection + cursor lists of a corpus side.
+Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap);
+// Per-tile bitmap tier + cursor records of a side, against corpus tiers.
+void build_tiles(Arena& ar, hipStream_t st, Side& s, const Tiers& tr, int32_t self);
+// Band scan + rescore; output pairs stay on the device.
+ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side& J,
+                            const Tiers& tr, const RunConfig& rc);
+
+// Full self-join into a result (host or device pairs per opts).
+int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
+
+// Sorting and copying helpers.
+void sort_pairs(apss_pair* p, int64_t n);
+inline int blocks_for(int64_t n, int threads, int cap = 1 << 20) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+}  // namespace apss

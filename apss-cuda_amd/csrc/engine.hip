@@ -1,0 +1,533 @@
+// Host orchestration of one APSS job on one GPU.
+//
+// Replaces the per-job sequence of similarity_join_device
+// (similarity_search.cu:1273-1343 for a cross join, :2650-2691 for a self
+// join): there, every job allocates a dense nX*nY float matrix (no memset),
+// launches five kernels with a host synchronisation after each and reads the
+// scan grid size back over PCIe.  Here a job is: validate + norms, feature
+// histogram, tier selection (one small device->host copy of the histogram),
+// cursor lists, per-tile bitmap tier, one band-scan launch over all tile pairs
+// and one FP64 rescore launch; the only other synchronisations read the pair
+// counters.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+#include "engine.h"
+
+namespace apss {
+
+// ---------------------------------------------------------------------------
+// Arena
+
+Arena::~Arena() {
+  for (auto& b : blocks_) (void)hipFree(b.first);
+}
+
+void* Arena::raw(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (budget_ > 0 && used_ + (int64_t)bytes > budget_)
+    throw Error{APSS_E_NOMEM, "device memory budget exceeded (" + std::to_string(used_ + bytes) +
+                                  " > " + std::to_string(budget_) + " bytes)"};
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    throw Error{APSS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+  }
+  blocks_.push_back({p, bytes});
+  used_ += (int64_t)bytes;
+  peak_ = std::max(peak_, used_);
+  return p;
+}
+
+void Arena::release(void* p) {
+  for (size_t k = 0; k < blocks_.size(); ++k)
+    if (blocks_[k].first == p) {
+      (void)hipFree(p);
+      used_ -= (int64_t)blocks_[k].second;
+      blocks_.erase(blocks_.begin() + (long)k);
+      return;
+    }
+}
+
+void* Arena::detach(void* p) {
+  for (size_t k = 0; k < blocks_.size(); ++k)
+    if (blocks_[k].first == p) {
+      used_ -= (int64_t)blocks_[k].second;
+      blocks_.erase(blocks_.begin() + (long)k);
+      return p;
+    }
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Timer
+
+Timer::Timer(hipStream_t st) : s(st) {
+  APSS_HIP(hipEventCreate(&a));
+  APSS_HIP(hipEventCreate(&b));
+}
+Timer::~Timer() {
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+}
+void Timer::start() { APSS_HIP(hipEventRecord(a, s)); }
+double Timer::stop_ms() {
+  APSS_HIP(hipEventRecord(b, s));
+  APSS_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  APSS_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+
+static void fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+  APSS_HIP(hipGetLastError());
+}
+static void fill_u64(hipStream_t st, unsigned long long* p, int64_t n, unsigned long long v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+  APSS_HIP(hipGetLastError());
+}
+
+TileSide Side::view() const {
+  TileSide v;
+  v.n = n;
+  v.T = T;
+  v.rp = rp;
+  v.col = col;
+  v.val = val;
+  v.norm = norm;
+  v.hnorm = hnorm;
+  v.bmp = bmp;
+  v.ptr = ptr;
+  v.base = dbase;
+  v.drow = drow;
+  v.dval = dval;
+  v.sbase = sbase;
+  v.srec = srec;
+  return v;
+}
+
+static std::string verr_text(uint32_t e) {
+  std::string s;
+  if (e & VERR_ROWPTR) s += " row_ptr not monotone or ends wrong;";
+  if (e & VERR_COL) s += " column id out of range;";
+  if (e & VERR_ORDER) s += " column ids not strictly ascending in a row;";
+  if (e & VERR_VALUE) s += " non-finite value;";
+  if (e & VERR_NORM) s += " row norm too large;";
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// prepare_side: upload + validation + norms
+
+Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
+  if (X.n_rows < 0 || X.n_cols < 0 || X.nnz < 0)
+    throw Error{APSS_E_INVALID, "negative matrix dimension"};
+  if (X.n_rows > 0 && (!X.row_ptr || (X.nnz > 0 && (!X.col || !X.val))))
+    throw Error{APSS_E_INVALID, "null CSR array"};
+  if (X.n_rows >= (int64_t)0x7fffffff || X.n_cols >= (int64_t)0x7fffffff)
+    throw Error{APSS_E_UNSUPPORTED, "more than 2^31-1 rows or columns"};
+  if (p < 0 || p > X.n_cols) throw Error{APSS_E_INVALID, "p outside [0, n_cols]"};
+  Side s;
+  s.n = X.n_rows;
+  s.nnz = X.nnz;
+  s.d = X.n_cols;
+  if (X.on_device) {
+    s.rp = X.row_ptr;
+    s.col = X.col;
+    s.val = X.val;
+  } else {
+    if (X.n_rows > 0 && X.row_ptr[0] != 0) throw Error{APSS_E_INVALID, "row_ptr[0] != 0"};
+    if (X.n_rows > 0 && X.row_ptr[X.n_rows] != X.nnz)
+      throw Error{APSS_E_INVALID, "row_ptr[n_rows] != nnz"};
+    int64_t* rp = ar.alloc<int64_t>(s.n + 1);
+    int32_t* col = ar.alloc<int32_t>(s.nnz);
+    float* val = ar.alloc<float>(s.nnz);
+    if (s.n >= 0) APSS_HIP(hipMemcpyAsync(rp, X.row_ptr, sizeof(int64_t) * (s.n + 1), hipMemcpyHostToDevice, st));
+    if (s.nnz > 0) {
+      APSS_HIP(hipMemcpyAsync(col, X.col, sizeof(int32_t) * s.nnz, hipMemcpyHostToDevice, st));
+      APSS_HIP(hipMemcpyAsync(val, X.val, sizeof(float) * s.nnz, hipMemcpyHostToDevice, st));
+    }
+    s.rp = rp;
+    s.col = col;
+    s.val = val;
+  }
+  s.norm = ar.alloc<float>(s.n);
+  s.hnorm = ar.alloc<float>(s.n);
+  uint32_t* err = ar.alloc<uint32_t>(1);
+  fill_i32(st, (int32_t*)err, 1, 0);
+  if (s.n > 0) {
+    hipLaunchKernelGGL(k_validate_norms, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n,
+                       s.d, s.nnz, s.rp, s.col, s.val, p, s.norm, s.hnorm, err);
+    APSS_HIP(hipGetLastError());
+  }
+  uint32_t herr = 0;
+  APSS_HIP(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipStreamSynchronize(st));
+  ar.release(err);
+  if (herr) throw Error{APSS_E_INVALID, "invalid CSR:" + verr_text(herr)};
+  // maxima for the global fast-reject bound
+  if (s.n > 0) {
+    std::vector<int64_t> hrp((size_t)s.n + 1);
+    std::vector<float> hn((size_t)s.n);
+    APSS_HIP(hipMemcpyAsync(hrp.data(), s.rp, sizeof(int64_t) * (s.n + 1), hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipMemcpyAsync(hn.data(), s.norm, sizeof(float) * s.n, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    int64_t ml = 0;
+    float mn = 0.f;
+    for (int64_t i = 0; i < s.n; ++i) {
+      ml = std::max(ml, hrp[i + 1] - hrp[i]);
+      mn = std::max(mn, hn[i]);
+    }
+    s.maxlen = (int32_t)ml;
+    s.maxnorm = mn;
+  }
+  s.T = (int32_t)((s.n + TILE - 1) / TILE);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// build_tiers: histogram, class per feature, cursor lists
+
+Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap) {
+  Tiers tr;
+  tr.d = s.d;
+  tr.p = p;
+  const int64_t d = s.d;
+  int32_t* cnt = ar.alloc<int32_t>(d + 1);
+  fill_i32(st, cnt, d + 1, 0);
+  if (s.nnz > 0) {
+    const int64_t chunk = 1 << 16;
+    hipLaunchKernelGGL(k_col_hist, dim3(blocks_for(s.nnz, (int)chunk, 1 << 30)), dim3(1024), 0, st,
+                       s.nnz, s.col, p, chunk, cnt);
+    APSS_HIP(hipGetLastError());
+  }
+  std::vector<int32_t> h((size_t)d + 1);
+  APSS_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int32_t) * (d + 1), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipStreamSynchronize(st));
+  ar.release(cnt);
+
+  const int32_t need = self ? 2 : 1;
+  const int64_t T = std::max<int64_t>(1, (s.n + TILE - 1) / TILE);
+  const int64_t lmin = std::max<int64_t>(need, T / 8);
+  int32_t cap = kd_cap > 0 ? std::min(kd_cap, KD_MAX) : KD_MAX;
+  std::vector<int32_t> cand;
+  int64_t n_forced = 0;
+  for (int64_t c = p; c < d; ++c) {
+    if (h[c] >= lmin || h[c] > S_LIST_MAX) cand.push_back((int32_t)c);
+    if (h[c] > S_LIST_MAX) ++n_forced;
+  }
+  if (n_forced > cap)
+    throw Error{APSS_E_UNSUPPORTED, std::to_string(n_forced) + " features have lists longer than " +
+                                        std::to_string(S_LIST_MAX) + " but the bitmap tier holds " +
+                                        std::to_string(cap)};
+  std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) { return h[x] > h[y]; });
+  if ((int64_t)cand.size() > cap) cand.resize(cap);
+  // bitmap-tier features ordered by id (keeps the tile blocks in id order)
+  std::sort(cand.begin(), cand.end());
+  std::vector<int32_t> fm((size_t)d, F_IGNORE);
+  for (int64_t c = 0; c < std::min<int64_t>(p, d); ++c) fm[c] = F_HEAD;
+  for (int64_t c = p; c < d; ++c)
+    if (h[c] >= need) fm[c] = F_CURSOR;
+  for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
+  tr.n_bitmap = (int32_t)cand.size();
+  tr.KD = std::max(64, (tr.n_bitmap + 63) / 64 * 64);
+  tr.KW = tr.KD / 64;
+  std::vector<int64_t> sp((size_t)d + 1, 0);
+  std::vector<int32_t> small_lists, large_lists;
+  for (int64_t c = 0; c < d; ++c) {
+    int64_t L = fm[c] == F_CURSOR ? h[c] : 0;
+    sp[c + 1] = sp[c] + L;
+    if (L >= 2 && L <= 64) small_lists.push_back((int32_t)c);
+    else if (L > 64) large_lists.push_back((int32_t)c);
+  }
+  tr.n_cursor_entries = sp[d];
+  if (tr.n_cursor_entries >= ((int64_t)1 << 31))
+    throw Error{APSS_E_UNSUPPORTED, "cursor tier larger than 2^31 entries"};
+  tr.fmap = ar.alloc<int32_t>(d);
+  tr.sptr = ar.alloc<int64_t>(d + 1);
+  tr.srow = ar.alloc<int32_t>(tr.n_cursor_entries);
+  tr.sval = ar.alloc<float>(tr.n_cursor_entries);
+  APSS_HIP(hipMemcpyAsync(tr.fmap, fm.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
+  APSS_HIP(hipMemcpyAsync(tr.sptr, sp.data(), sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
+  if (tr.n_cursor_entries > 0) {
+    unsigned long long* cur = ar.alloc<unsigned long long>(d + 1);
+    hipLaunchKernelGGL(k_copy_i64, dim3(blocks_for(d + 1, 256, 4096)), dim3(256), 0, st,
+                       (int64_t*)cur, tr.sptr, d + 1);
+    hipLaunchKernelGGL(k_cursor_scatter, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n,
+                       (int64_t)0, s.rp, s.col, s.val, tr.fmap, cur, tr.srow, tr.sval);
+    APSS_HIP(hipGetLastError());
+    int32_t* lists = ar.alloc<int32_t>((int64_t)(small_lists.size() + large_lists.size()));
+    if (!small_lists.empty()) {
+      APSS_HIP(hipMemcpyAsync(lists, small_lists.data(), sizeof(int32_t) * small_lists.size(),
+                              hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_sort_small, dim3(blocks_for((int64_t)small_lists.size(), 4, 1 << 30)),
+                         dim3(256), 0, st, lists, (int32_t)small_lists.size(), tr.sptr, tr.srow,
+                         tr.sval);
+      APSS_HIP(hipGetLastError());
+    }
+    if (!large_lists.empty()) {
+      int32_t* ll = lists + small_lists.size();
+      APSS_HIP(hipMemcpyAsync(ll, large_lists.data(), sizeof(int32_t) * large_lists.size(),
+                              hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_sort_large, dim3((unsigned)large_lists.size()), dim3(256), 0, st, ll,
+                         tr.sptr, tr.srow, tr.sval);
+      APSS_HIP(hipGetLastError());
+    }
+    APSS_HIP(hipStreamSynchronize(st));
+    ar.release(lists);
+    ar.release(cur);
+  }
+  return tr;
+}
+
+// ---------------------------------------------------------------------------
+// build_tiles
+
+void build_tiles(Arena& ar, hipStream_t st, Side& s, const Tiers& tr, int32_t self) {
+  const int32_t T = s.T;
+  s.ptr = ar.alloc<int32_t>((int64_t)T * (tr.KD + 1));
+  s.bmp = ar.alloc<uint64_t>((int64_t)T * tr.KW);
+  s.dbase = ar.alloc<int64_t>((int64_t)T + 1);
+  s.sbase = ar.alloc<int64_t>((int64_t)T + 1);
+  fill_i32(st, s.ptr, (int64_t)T * (tr.KD + 1), 0);
+  BuildSide b{};
+  b.n = s.n;
+  b.T = T;
+  b.KD = tr.KD;
+  b.p = tr.p;
+  b.self = self;
+  b.row_off = 0;
+  b.rp = s.rp;
+  b.col = s.col;
+  b.val = s.val;
+  b.fmap = tr.fmap;
+  b.ptr = s.ptr;
+  b.bmp = s.bmp;
+  b.dtot = s.dbase;
+  b.stot = s.sbase;
+  b.sptr = tr.sptr;
+  b.srow = tr.srow;
+  if (T > 0) {
+    hipLaunchKernelGGL(k_tile_count, dim3(T), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, s.dbase, (int64_t)T);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, s.sbase, (int64_t)T);
+    hipLaunchKernelGGL(k_tile_ptr_scan, dim3(T), dim3(1024), 0, st, s.ptr, s.bmp, tr.KD);
+    APSS_HIP(hipGetLastError());
+  } else {
+    fill_u64(st, (unsigned long long*)s.dbase, 1, 0);
+    fill_u64(st, (unsigned long long*)s.sbase, 1, 0);
+  }
+  int64_t tot[2] = {0, 0};
+  APSS_HIP(hipMemcpyAsync(&tot[0], s.dbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipMemcpyAsync(&tot[1], s.sbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipStreamSynchronize(st));
+  s.nD = tot[0];
+  s.nS = tot[1];
+  if (s.nD >= ((int64_t)1 << 31))
+    throw Error{APSS_E_UNSUPPORTED, "bitmap tier larger than 2^31 entries"};
+  s.drow = ar.alloc<uint8_t>(s.nD);
+  s.dval = ar.alloc<float>(s.nD);
+  s.srec = ar.alloc<int4>(s.nS);
+  b.drow = s.drow;
+  b.dval = s.dval;
+  b.srec = s.srec;
+  if (T > 0) {
+    hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
+    APSS_HIP(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scan + rescore
+
+ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side& J,
+                            const Tiers& tr, const RunConfig& rc) {
+  ScanOutput so;
+  const int64_t maxlen = std::max(I.maxlen, J.maxlen);
+  const double maxn = (double)std::max(I.maxnorm, 1e-30f) * (double)std::max(J.maxnorm, 1e-30f);
+  const double eps_max = gamma32(maxlen + 4) * maxn * (1.0 + 1e-6) + 1e-44 * (maxlen + 4);
+  float t_lo = (float)(rc.t - 2.0 * eps_max);
+  if ((double)t_lo > rc.t - 2.0 * eps_max) t_lo = nextafterf(t_lo, -INFINITY);
+
+  // pair-space size bounds the output capacity
+  const double cells = rc.self ? 0.5 * (double)I.n * (double)(I.n - 1) : (double)I.n * (double)J.n;
+  size_t free_b = 0, total_b = 0;
+  APSS_HIP(hipMemGetInfo(&free_b, &total_b));
+  int64_t cap = (int64_t)std::min<double>(cells, (double)free_b * 0.25 / sizeof(apss_pair));
+  cap = std::max<int64_t>(cap, 1 << 16);
+  int64_t bcap = std::max<int64_t>(1 << 20, cap / 64);
+  unsigned long long* ctr = ar.alloc<unsigned long long>(8);
+  int32_t* spill = ar.alloc<int32_t>(2 * std::max<int64_t>(I.nS, 1));
+  Timer tm(st);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    apss_pair* out = ar.alloc<apss_pair>(cap);
+    unsigned long long* band = ar.alloc<unsigned long long>(bcap);
+    fill_u64(st, ctr, 8, 0);
+    ScanArgs a{};
+    a.I = I.view();
+    a.J = J.view();
+    a.S.ptr = tr.sptr;
+    a.S.row = tr.srow;
+    a.S.val = tr.sval;
+    a.KD = tr.KD;
+    a.KW = tr.KW;
+    a.p = tr.p;
+    a.self = rc.self;
+    a.head_mode = tr.p > 0 ? 1 : 0;
+    a.t = rc.t;
+    a.t_lo = t_lo;
+    a.i_off = rc.i_off;
+    a.j_off = rc.j_off;
+    a.out = out;
+    a.out_cnt = ctr + 0;
+    a.out_cap = cap;
+    a.band = band;
+    a.band_cnt = ctr + 1;
+    a.band_cap = bcap;
+    a.rescore_all = rc.rescore_all;
+    a.band_first = 0;
+    a.band_count = I.T;
+    a.stats = ctr + 2;
+    a.snext_spill = spill;
+    tm.start();
+    if (I.T > 0 && J.T > 0) {
+      hipLaunchKernelGGL(k_band_scan, dim3(I.T), dim3(SCAN_THREADS), 0, st, a);
+      APSS_HIP(hipGetLastError());
+    }
+    so.ms_scan = tm.stop_ms();
+    unsigned long long h[8];
+    APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    const int64_t nout = (int64_t)h[0], nband = (int64_t)h[1];
+    if (nout > cap || nband > bcap) {  // capacity too small: grow and run again
+      ar.release(out);
+      ar.release(band);
+      cap = std::max(cap, nout + nband + 1024);
+      bcap = std::max(bcap, nband + 1024);
+      continue;
+    }
+    so.products = (int64_t)h[2];
+    so.head_pairs = (int64_t)h[3];
+    so.cells = (int64_t)h[4];
+    so.band = nband;
+    if (nout + nband > cap) {  // room for every band pair before the rescore
+      apss_pair* bigger = ar.alloc<apss_pair>(nout + nband);
+      if (nout) APSS_HIP(hipMemcpyAsync(bigger, out, sizeof(apss_pair) * nout, hipMemcpyDeviceToDevice, st));
+      ar.release(out);
+      out = bigger;
+      cap = nout + nband;
+    }
+    tm.start();
+    if (nband > 0) {
+      hipLaunchKernelGGL(k_rescore, dim3(blocks_for(nband, 256, 1 << 30)), dim3(256), 0, st, band,
+                         nband, I.rp, I.col, I.val, J.rp, J.col, J.val, rc.t, rc.i_off, rc.j_off,
+                         out, ctr + 0, cap);
+      APSS_HIP(hipGetLastError());
+    }
+    so.ms_rescore = tm.stop_ms();
+    APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    so.count = (int64_t)h[0];
+    so.pairs = out;
+    ar.release(band);
+    return so;
+  }
+  throw Error{APSS_E_NOMEM, "output capacity could not be sized"};
+}
+
+// ---------------------------------------------------------------------------
+// run_self
+
+void sort_pairs(apss_pair* p, int64_t n) {
+  std::sort(p, p + n, [](const apss_pair& x, const apss_pair& y) {
+    return x.i != y.i ? x.i < y.i : x.j < y.j;
+  });
+}
+
+struct DevicePairs {  // result.internal_ for device-resident output
+  void* ptr;
+};
+
+int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out) {
+  apss_opts def;
+  apss_opts_init(&def);
+  if (!o) o = &def;
+  if (!X || !out) throw Error{APSS_E_INVALID, "null argument"};
+  if (!(t == t) || std::isinf(t)) throw Error{APSS_E_INVALID, "t must be finite"};
+  std::memset(out, 0, sizeof(*out));
+  APSS_HIP(hipSetDevice(o->device));
+  hipStream_t st = (hipStream_t)o->stream;
+  bool own_stream = false;
+  if (!st) {
+    APSS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    own_stream = true;
+  }
+  struct StreamGuard {
+    hipStream_t s;
+    bool own;
+    ~StreamGuard() { if (own) (void)hipStreamDestroy(s); }
+  } sg{st, own_stream};
+  auto t0 = std::chrono::steady_clock::now();
+  Arena ar(o->device, o->mem_budget);
+  if (p < 0) p = 0;  // autotune picks p = 0 for the FP32 path (all-tail scan)
+  Timer tm(st);
+  tm.start();
+  Side s = prepare_side(ar, st, *X, p);
+  out->ms_upload = tm.stop_ms();
+  tm.start();
+  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
+  build_tiles(ar, st, s, tr, 1);
+  out->ms_index = tm.stop_ms();
+  RunConfig rc;
+  rc.t = t;
+  rc.p = p;
+  rc.self = 1;
+  rc.rescore_all = o->rescore_all;
+  ScanOutput so = scan_and_rescore(ar, st, s, s, tr, rc);
+  out->ms_scan = so.ms_scan;
+  out->ms_rescore = so.ms_rescore;
+  out->count = so.count;
+  out->p_used = p;
+  out->cells = so.cells;
+  out->band = so.band;
+  out->products = so.products;
+  out->head_pairs = so.head_pairs;
+  tm.start();
+  if (o->output_on_device && !o->sort_output) {
+    out->pairs = (apss_pair*)ar.detach(so.pairs);
+    out->on_device = 1;
+  } else {
+    apss_pair* h = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(so.count, 1));
+    if (!h) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+    if (so.count)
+      APSS_HIP(hipMemcpyAsync(h, so.pairs, sizeof(apss_pair) * so.count, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    if (o->sort_output) sort_pairs(h, so.count);
+    if (o->output_on_device) {
+      apss_pair* d = so.pairs;
+      APSS_HIP(hipMemcpyAsync(d, h, sizeof(apss_pair) * so.count, hipMemcpyHostToDevice, st));
+      APSS_HIP(hipStreamSynchronize(st));
+      std::free(h);
+      out->pairs = (apss_pair*)ar.detach(d);
+      out->on_device = 1;
+    } else {
+      out->pairs = h;
+      out->on_device = 0;
+    }
+  }
+  out->ms_download = tm.stop_ms();
+  APSS_HIP(hipStreamSynchronize(st));
+  out->ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return APSS_OK;
+}
+
+}  // namespace apss

@@ -1,0 +1,343 @@
+// Kernel (a): inverted-index construction (CSR -> tiled CSC) for gfx950.
+//
+// The reference builds its tail index with a global-atomic histogram per
+// partition (generate_index_ptr4, similarity_search.cu:105-191), a host scan
+// (:1168-1186) and an atomic-cursor scatter into one list per feature
+// (construct_Index2, :221-274), then re-uploads the pointers to undo the
+// cursor (:1256-1263).  Here the tail (ids >= p) is split by list length:
+//
+//   bitmap tier  features whose lists are long (present in a sizeable share of
+//                the row tiles): stored per row tile as a CSC block with a
+//                presence bitmap and segment pointers, so a tile pair finds
+//                its common features by AND-ing two bitmaps;
+//   cursor tier  short lists: one global CSC, ascending row per list, walked
+//                by per-entry cursors while a band of tiles sweeps its
+//                partner tiles.
+//
+// Histogram: LDS-privatised per workgroup for the most frequent ids, global
+// atomics for the rest.  Scan: per tile in LDS.  Scatter: atomic cursors whose
+// final values are the segment ends, so the pointer array is written shifted by
+// one slot and comes out right without a restore pass.
+#include "apss_internal.h"
+
+namespace apss {
+
+// ---------------------------------------------------------------------------
+// fills (used instead of hipMemsetAsync so that the sequence can be captured)
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_copy_i64(int64_t* dst, const int64_t* src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// validation + norms: one wave per row
+
+__device__ __forceinline__ float round_up_f(double s) {
+  float f = (float)s;
+  if ((double)f < s) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+__global__ __launch_bounds__(256) void k_validate_norms(int64_t n, int64_t d, int64_t nnz,
+                                                        const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val, int32_t p,
+                                                        float* __restrict__ norm,
+                                                        float* __restrict__ hnorm,
+                                                        uint32_t* __restrict__ err) {
+  const int lane = lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
+    int64_t a = rp[r], b = rp[r + 1];
+    uint32_t e_bits = 0;
+    if (r == 0 && a != 0) e_bits |= VERR_ROWPTR;
+    if (r == n - 1 && b != nnz) e_bits |= VERR_ROWPTR;
+    if (a < 0 || b > nnz || a > b) {
+      if (lane == 0) atomicOr(err, e_bits | VERR_ROWPTR);
+      if (lane == 0) { norm[r] = 0.f; hnorm[r] = 0.f; }
+      continue;
+    }
+    double ss = 0.0, hs = 0.0;
+    for (int64_t e = a + lane; e < b; e += 64) {
+      int32_t c = col[e];
+      float v = val[e];
+      if (c < 0 || c >= d) e_bits |= VERR_COL;
+      if (e + 1 < b && col[e + 1] <= c) e_bits |= VERR_ORDER;
+      if (!isfinite(v)) e_bits |= VERR_VALUE;
+      double v2 = (double)v * (double)v;
+      ss += v2;
+      if (c < p) hs += v2;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      ss += __shfl_xor(ss, o);
+      hs += __shfl_xor(hs, o);
+      e_bits |= __shfl_xor(e_bits, o);
+    }
+    if (lane == 0) {
+      // rows must keep |score| <= ||x|| ||y|| far from FP32 overflow
+      if (!(ss < 1.3e36)) e_bits |= VERR_NORM;
+      norm[r] = round_up_f(sqrt(ss));
+      hnorm[r] = round_up_f(sqrt(hs));
+      if (e_bits) atomicOr(err, e_bits);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// feature histogram of ids >= p (LDS-privatised for ids < HIST_LDS)
+
+constexpr int HIST_LDS = 16384;
+
+__global__ __launch_bounds__(1024) void k_col_hist(int64_t nnz, const int32_t* __restrict__ col,
+                                                   int32_t p, int64_t chunk,
+                                                   int32_t* __restrict__ cnt) {
+  __shared__ int32_t h[HIST_LDS];
+  for (int i = threadIdx.x; i < HIST_LDS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  int64_t lo = blockIdx.x * chunk, hi = lo + chunk < nnz ? lo + chunk : nnz;
+  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+    int32_t c = col[e];
+    if (c < p) continue;
+    if (c < HIST_LDS) atomicAdd(&h[c], 1);
+    else atomicAdd(&cnt[c], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HIST_LDS; i += blockDim.x)
+    if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+
+// ---------------------------------------------------------------------------
+// cursor tier: scatter into one list per feature, then sort each list by row
+
+__global__ __launch_bounds__(256) void k_cursor_scatter(int64_t n, int64_t row_off,
+                                                        const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val,
+                                                        const int32_t* __restrict__ fmap,
+                                                        unsigned long long* __restrict__ cur,
+                                                        int32_t* __restrict__ srow,
+                                                        float* __restrict__ sval) {
+  const int lane = lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
+    for (int64_t e = rp[r] + lane; e < rp[r + 1]; e += 64) {
+      int32_t c = col[e];
+      if (fmap[c] != F_CURSOR) continue;
+      unsigned long long q = atomicAdd(&cur[c], 1ull);
+      srow[q] = (int32_t)(r + row_off);
+      sval[q] = val[e];
+    }
+  }
+}
+
+// One wave sorts one list of length <= 64 (bitonic network over lanes).
+__global__ __launch_bounds__(256) void k_sort_small(const int32_t* __restrict__ lists, int32_t nl,
+                                                    const int64_t* __restrict__ sptr,
+                                                    int32_t* __restrict__ srow,
+                                                    float* __restrict__ sval) {
+  int w = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (w >= nl) return;
+  const int lane = lane_id();
+  int32_t f = lists[w];
+  int64_t a = sptr[f];
+  int len = (int)(sptr[f + 1] - a);
+  int32_t key = lane < len ? srow[a + lane] : INT32_MAX;
+  float v = lane < len ? sval[a + lane] : 0.f;
+  for (int k = 2; k <= 64; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      int32_t ok = __shfl_xor(key, j);
+      float ov = __shfl_xor(v, j);
+      bool up = ((lane & k) == 0);
+      bool lower = ((lane & j) == 0);
+      bool take_min = (lower == up);
+      bool swap = take_min ? (ok < key) : (ok > key);
+      if (swap) { key = ok; v = ov; }
+    }
+  }
+  if (lane < len) { srow[a + lane] = key; sval[a + lane] = v; }
+}
+
+// One workgroup sorts one list of length <= S_LIST_MAX in LDS.
+__global__ __launch_bounds__(256) void k_sort_large(const int32_t* __restrict__ lists,
+                                                    const int64_t* __restrict__ sptr,
+                                                    int32_t* __restrict__ srow,
+                                                    float* __restrict__ sval) {
+  __shared__ int32_t key[S_LIST_MAX];
+  __shared__ float v[S_LIST_MAX];
+  int32_t f = lists[blockIdx.x];
+  int64_t a = sptr[f];
+  int len = (int)(sptr[f + 1] - a);
+  int np = 64;
+  while (np < len) np <<= 1;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    key[i] = i < len ? srow[a + i] : INT32_MAX;
+    v[i] = i < len ? sval[a + i] : 0.f;
+  }
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          bool up = ((i & k) == 0);
+          int32_t ki = key[i], kj = key[ixj];
+          if ((ki > kj) == up) {
+            key[i] = kj; key[ixj] = ki;
+            float t = v[i]; v[i] = v[ixj]; v[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) { srow[a + i] = key[i]; sval[a + i] = v[i]; }
+}
+
+// ---------------------------------------------------------------------------
+// per-tile bitmap tier
+
+
+
+// Count bitmap-tier entries per (tile, feature) into slot f+1 of the tile's
+// pointer row and the tile totals of both tiers.
+__global__ __launch_bounds__(256) void k_tile_count(BuildSide s) {
+  __shared__ unsigned long long dcount, scount;
+  const int tile = blockIdx.x;
+  if (threadIdx.x == 0) { dcount = 0; scount = 0; }
+  __syncthreads();
+  int64_t r0 = (int64_t)tile * TILE, r1 = r0 + TILE < s.n ? r0 + TILE : s.n;
+  int32_t* prow = s.ptr + (int64_t)tile * (s.KD + 1);
+  unsigned long long dc = 0, sc = 0;
+  for (int64_t e = s.rp[r0] + threadIdx.x; e < s.rp[r1]; e += blockDim.x) {
+    int32_t c = s.col[e];
+    if (c < s.p) continue;
+    int32_t f = s.fmap[c];
+    if (f >= 0) { atomicAdd(&prow[f + 1], 1); ++dc; }
+    else if (f == F_CURSOR) ++sc;
+  }
+  atomicAdd(&dcount, dc);
+  atomicAdd(&scount, sc);
+  __syncthreads();
+  if (threadIdx.x == 0) { s.dtot[tile] = (int64_t)dcount; s.stot[tile] = (int64_t)scount; }
+}
+
+// Exclusive scan of a[0..n) in place, a[n] = total (single workgroup).
+__global__ __launch_bounds__(1024) void k_scan_i64(int64_t* a, int64_t n) {
+  __shared__ int64_t part[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int64_t x = i < n ? a[i] : 0;
+    part[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int64_t y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < n) a[i] = carry + part[threadIdx.x] - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a[n] = carry;
+}
+
+// Per tile: counts c[f] (slot f+1) -> exclusive starts E[f] (slot f+1), slot 0
+// = 0, plus the presence bitmap.  After the scatter, which advances slot f+1
+// from E[f] to E[f]+c[f] = E[f+1], the row is the standard pointer array.
+__global__ __launch_bounds__(1024) void k_tile_ptr_scan(int32_t* ptr, uint64_t* bmp, int32_t KD) {
+  __shared__ int32_t part[1024];
+  __shared__ int32_t carry;
+  const int tile = blockIdx.x;
+  int32_t* prow = ptr + (int64_t)tile * (KD + 1);
+  uint64_t* brow = bmp + (int64_t)tile * (KD / 64);
+  if (threadIdx.x == 0) { carry = 0; prow[0] = 0; }
+  __syncthreads();
+  for (int32_t base = 0; base < KD; base += 1024) {
+    int32_t f = base + threadIdx.x;
+    int32_t x = f < KD ? prow[f + 1] : 0;
+    unsigned long long m = __ballot(x > 0);
+    if (lane_id() == 0 && f < KD) brow[f / 64] = m;
+    part[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int32_t y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (f < KD) prow[f + 1] = carry + part[threadIdx.x] - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo, int64_t hi,
+                                                   int32_t key) {
+  while (lo < hi) {
+    int64_t m = (lo + hi) >> 1;
+    if (a[m] < key) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+
+// Scatter bitmap-tier entries into their tile blocks and write the cursor
+// records of this side's cursor-tier entries.
+__global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
+  __shared__ int32_t rps[TILE + 1];
+  __shared__ unsigned int sfill;
+  const int tile = blockIdx.x;
+  int64_t r0 = (int64_t)tile * TILE, r1 = r0 + TILE < s.n ? r0 + TILE : s.n;
+  int nr = (int)(r1 - r0);
+  int64_t e0 = s.rp[r0];
+  for (int i = threadIdx.x; i <= nr; i += blockDim.x) rps[i] = (int32_t)(s.rp[r0 + i] - e0);
+  if (threadIdx.x == 0) sfill = 0;
+  __syncthreads();
+  int32_t* prow = s.ptr + (int64_t)tile * (s.KD + 1);
+  int64_t dbase = s.dtot[tile], sbase = s.stot[tile];
+  int32_t ne = rps[nr];
+  for (int32_t k = threadIdx.x; k < ne; k += blockDim.x) {
+    int64_t e = e0 + k;
+    int32_t c = s.col[e];
+    if (c < s.p) continue;
+    int32_t f = s.fmap[c];
+    if (f < 0 && f != F_CURSOR) continue;
+    // local row of entry k: last r with rps[r] <= k
+    int lo = 0, hi = nr;
+    while (hi - lo > 1) {
+      int m = (lo + hi) >> 1;
+      if (rps[m] <= k) lo = m; else hi = m;
+    }
+    if (f >= 0) {
+      int32_t q = atomicAdd(&prow[f + 1], 1);
+      s.drow[dbase + q] = (uint8_t)lo;
+      s.dval[dbase + q] = s.val[e];
+    } else {
+      unsigned int q = atomicAdd(&sfill, 1u);
+      int64_t a = s.sptr[c], b = s.sptr[c + 1];
+      int64_t start = s.self ? lower_bound_i32(s.srow, a, b, (int32_t)(s.row_off + r0 + lo + 1)) : a;
+      s.srec[sbase + q] = make_int4(lo, __float_as_int(s.val[e]), (int)start, (int)b);
+    }
+  }
+}
+
+}  // namespace apss

@@ -1,0 +1,178 @@
+/*
+ * apss.h -- C-ABI of the MI355X (gfx950) all-pairs similarity search library
+ * (libapss.so).  Plain C types only; no HIP or torch types in any signature.
+ *
+ * The reference exposes exactly one entry point,
+ *   void similarity_join_device(int* dim_A, float* feature_A, ... 70 args)
+ * (similarity_search.h:3-14, defined at similarity_search.cu:934-3015), with a
+ * compile-time threshold (similarity_search.cu:18), a literal head/tail split
+ * of 4096 and no output.  This header keeps that symbol unchanged (bottom of
+ * the file) and adds the runtime-parameterised interface it implies: a CSR
+ * matrix, a threshold t and a split p in, (i, j, score) pairs out.
+ *
+ * Semantics shared by every entry point (reference hybrid, similarity_search.cu
+ * :370-437, 441-506, 510-531, 642-735, 833-915):
+ *   - rows are sparse vectors; a score is the raw dot product sum_d x_id*x_jd
+ *     (the reference never normalises; inputs are expected L2-normalised);
+ *   - the result is every pair i < j with score >= t (self-join), or every
+ *     (query, corpus) pair with score >= t (index queries);
+ *   - feature ids < p form the head (forward-list dot product, or BF16 MFMA
+ *     tiles in APSS_PREC_BF16_HEAD mode), ids >= p the tail (inverted lists);
+ *     the emitted pair set does not depend on p;
+ *   - the pair set equals the exact FP64 result (products exact in double,
+ *     summed in ascending feature id): every pair whose fast-path score lies
+ *     within its rigorous rounding bound of t is rescored in FP64 on the GPU;
+ *   - errors are returned as negative APSS_E_* codes; nothing calls exit().
+ */
+#ifndef APSS_H
+#define APSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APSS_VERSION_MAJOR 0
+#define APSS_VERSION_MINOR 1
+#define APSS_VERSION_PATCH 0
+
+enum {
+  APSS_OK = 0,
+  APSS_E_INVALID = -1,     /* malformed CSR, bad t/p/option */
+  APSS_E_NOMEM = -2,       /* allocation failed or memory budget exceeded */
+  APSS_E_DEVICE = -3,      /* HIP runtime error */
+  APSS_E_UNSUPPORTED = -4, /* input outside the supported envelope */
+  APSS_E_CANCELLED = -5,   /* request cancelled before completion */
+  APSS_E_BUSY = -6         /* serving admission refused (queue full / over budget) */
+};
+
+enum { APSS_PREC_FP32 = 0, APSS_PREC_BF16_HEAD = 1 };
+
+/* CSR matrix: row_ptr[n_rows+1] (row_ptr[0]=0, nondecreasing), col[nnz]
+ * strictly ascending inside each row and < n_cols, val[nnz] finite.
+ * on_device=1 means the three arrays are device pointers on opts.device.
+ * Replaces the SELL-32 arrays dim_X/feature_X/tile_start_X that main.cu
+ * builds per partition (main.cu:286-339). */
+typedef struct {
+  int64_t n_rows;
+  int64_t n_cols;
+  int64_t nnz;
+  const int64_t* row_ptr;
+  const int32_t* col;
+  const float* val;
+  int32_t on_device;
+  int32_t reserved_;
+} apss_csr;
+
+/* One result record (12 bytes).  For self-joins i < j are row indices; for
+ * index queries i is the query row and j the corpus row. */
+typedef struct {
+  uint32_t i;
+  uint32_t j;
+  float score;
+} apss_pair;
+
+typedef struct {
+  int32_t device;          /* HIP device ordinal (default 0) */
+  int32_t precision;       /* APSS_PREC_FP32 (default) or APSS_PREC_BF16_HEAD */
+  int32_t sort_output;     /* 1: pairs sorted by (i, j) */
+  int32_t rescore_all;     /* 1: every emitted score is the FP64 rescore */
+  int32_t output_on_device;/* 1: result.pairs is device memory (free with apss_result_free) */
+  int32_t dense_features;  /* cap of the bitmap tier (0 = automatic) */
+  int64_t mem_budget;      /* device bytes a call may allocate (0 = no limit) */
+  void* stream;            /* hipStream_t to run on (NULL = library-owned stream) */
+  int64_t reserved_[4];
+} apss_opts;
+
+typedef struct {
+  int64_t count;           /* number of result pairs */
+  apss_pair* pairs;        /* library-allocated; release with apss_result_free */
+  int32_t on_device;
+  int32_t p_used;          /* split actually used (after autotune) */
+  /* counters */
+  int64_t cells;           /* pair cells evaluated on the GPU */
+  int64_t band;            /* pairs sent to the FP64 rescore */
+  int64_t products;        /* tail partial products accumulated */
+  int64_t head_pairs;      /* pairs that ran the head forward-list dot */
+  /* timings in milliseconds (hipEvent on the run's stream) */
+  double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
+  void* internal_;
+} apss_result;
+
+typedef void (*apss_emit_fn)(const apss_pair* pairs, size_t count, void* user);
+
+const char* apss_version(void);
+const char* apss_strerror(int code);
+/* Detail message of the last failed call on this thread. */
+const char* apss_last_error(void);
+void apss_opts_init(apss_opts* o);
+
+/* Self-join APSS of X at threshold t with head/tail split p (p = -1: autotune).
+ * Replaces similarity_join_device (similarity_search.h:3-14) for one matrix. */
+int apss_run(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
+
+/* Streaming variant: pairs are handed to emit() in chunks on the calling
+ * thread instead of being returned; out (optional) receives the counters. */
+int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_emit_fn emit,
+                void* user, apss_result* out);
+
+/* Many independent instances in one batched launch sequence (one stream,
+ * one tile kernel over all instances' tiles).  out[k] receives instance k. */
+int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
+                     const apss_opts* o, apss_result* out);
+
+void apss_result_free(apss_result* r);
+
+/* Serving: a corpus index resident in device memory. */
+typedef struct apss_index apss_index;
+int apss_index_create(const apss_csr* corpus, int32_t p, const apss_opts* o, apss_index** out);
+/* Append rows to the corpus; they get ids n_rows_before .. n_rows_before+k-1. */
+int apss_index_insert(apss_index* ix, const apss_csr* rows);
+/* All (query, corpus) pairs with score >= t; safe to call from several threads. */
+int apss_index_query(apss_index* ix, const apss_csr* queries, double t, apss_result* out);
+int64_t apss_index_rows(const apss_index* ix);
+int apss_index_destroy(apss_index* ix);
+
+/* Continuous-batching server over an index: requests are admitted by memory
+ * budget and run concurrently on `slots` streams. */
+typedef struct apss_server apss_server;
+int apss_server_create(apss_index* ix, int32_t slots, int64_t mem_budget, int32_t max_queue,
+                       apss_server** out);
+/* Enqueue a query batch (copied); returns a ticket >= 0 or a negative code. */
+int64_t apss_server_submit(apss_server* s, const apss_csr* queries, double t);
+/* Wait for a ticket; fills out and the request's latency (submit->done, ms). */
+int apss_server_wait(apss_server* s, int64_t ticket, apss_result* out, double* latency_ms);
+int apss_server_cancel(apss_server* s, int64_t ticket);
+int apss_server_destroy(apss_server* s);
+
+/* Drop-in for the reference's only entry point (similarity_search.h:3-14):
+ * the same 70 arguments (8 SELL-32 partitions, their head norms, sizes and
+ * the three lookup tables).  The SELL-32 slices are converted to one CSR
+ * matrix (padding -1 dropped, rows in partition order A..H), then apss_run
+ * runs with the reference constants t = 0.3f (similarity_search.cu:18) and
+ * p = 4096.  Like the reference it prints a summary and returns nothing; the
+ * pair count of the last call is available from apss_compat_last_count(). */
+void similarity_join_device(int* dim_A, float* feature_A, int* tile_start_A, int* dim_B,
+                            float* feature_B, int* tile_start_B, int* dim_C, float* feature_C,
+                            int* tile_start_C, int* dim_D, float* feature_D, int* tile_start_D,
+                            int* dim_E, float* feature_E, int* tile_start_E, int* dim_F,
+                            float* feature_F, int* tile_start_F, int* dim_G, float* feature_G,
+                            int* tile_start_G, int* dim_H, float* feature_H, int* tile_start_H,
+                            float* pre_norm_A1, float* pre_norm_B1, float* pre_norm_C1,
+                            float* pre_norm_D1, float* pre_norm_E1, float* pre_norm_F1,
+                            float* pre_norm_G1, float* pre_norm_H1, int sum_vec_A, int sum_feat_A,
+                            int sum_tile_A, int sum_vec_B, int sum_feat_B, int sum_tile_B,
+                            int sum_vec_C, int sum_feat_C, int sum_tile_C, int sum_vec_D,
+                            int sum_feat_D, int sum_tile_D, int sum_vec_E, int sum_feat_E,
+                            int sum_tile_E, int sum_vec_F, int sum_feat_F, int sum_tile_F,
+                            int sum_vec_G, int sum_feat_G, int sum_tile_G, int sum_vec_H,
+                            int sum_feat_H, int sum_tile_H, int* b_id_a, int* apss_block_seg,
+                            int* apss_seg_start);
+int64_t apss_compat_last_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

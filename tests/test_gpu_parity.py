@@ -1,0 +1,140 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 oracle.
+
+Exact pair-set equality and scores within 1e-6 relative (the tolerance of the
+north-star contract), on seeded synthetic matrices at sizes the oracle
+finishes in seconds, plus the edge cases of the method.
+"""
+import numpy as np
+import pytest
+
+import apss
+import oracle
+from helpers import check_pairs, subset, tiny_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def run_and_check(rp, col, val, d, t, p=0, **opts):
+    got, info = apss.run(rp, col, val, d, t, p, **opts)
+    exp = oracle.apss_fp64(rp, col, val, d, t)
+    err = check_pairs(got, exp)
+    return got, exp, info, err
+
+
+def test_tiny_closed_forms():
+    # r0 == r1 (s = 1), r2 disjoint, r3 shares one feature with r0
+    x = 0.6, 0.8
+    rows = [{1: x[0], 7: x[1]}, {1: x[0], 7: x[1]}, {3: 1.0}, {7: 1.0}, {}]
+    rp, col, val, d = tiny_matrix(rows, 10)
+    got, info = apss.run(rp, col, val, d, 0.5, 0, sort_output=1)
+    pairs = [(int(a), int(b)) for a, b in zip(got["i"], got["j"])]
+    assert pairs == [(0, 1), (0, 3), (1, 3)]
+    s01 = float(np.float32(0.6) * np.float32(0.6)) + float(np.float32(0.8) * np.float32(0.8))
+    assert abs(got["score"][0] - s01) <= 1e-6 * s01
+    assert got["score"][1] == np.float32(0.8) and got["score"][2] == np.float32(0.8)
+
+
+def test_threshold_equal_to_score_is_included():
+    rp, col, val, d = tiny_matrix([{2: 0.6, 5: 0.8}, {2: 0.8, 5: 0.6}, {5: 1.0}], 8)
+    s = oracle.pair_fp64(rp, col, val, 0, 1)   # exact FP64 score of (0,1)
+    got, _ = apss.run(rp, col, val, d, s, 0)
+    assert (0, 1) in set(zip(got["i"].tolist(), got["j"].tolist()))
+    nxt = np.nextafter(s, 2.0)
+    got2, _ = apss.run(rp, col, val, d, nxt, 0)
+    assert (0, 1) not in set(zip(got2["i"].tolist(), got2["j"].tolist()))
+
+
+def test_empty_and_degenerate():
+    rp, col, val, d = tiny_matrix([], 5)
+    got, info = apss.run(rp, col, val, d, 0.1, 0)
+    assert len(got) == 0
+    rp, col, val, d = tiny_matrix([{}, {}, {1: 1.0}], 5)
+    got, info = apss.run(rp, col, val, d, 0.1, 0)
+    assert len(got) == 0
+    rp, col, val, d = tiny_matrix([{1: 1.0}, {1: 1.0}], 5)
+    got, info = apss.run(rp, col, val, d, 0.1, 5)  # all-head
+    assert len(got) == 1
+
+
+@pytest.mark.parametrize("t", [0.3, 0.5, 0.7])
+@pytest.mark.parametrize("p", [0, 64, 4096])
+def test_small_config_subset(t, p):
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 3000)
+    got, exp, info, err = run_and_check(rp, col, val, d, t, p)
+    assert info["p_used"] == p
+    assert len(exp) > 0
+
+
+@pytest.mark.parametrize("cap", [1, 64, 512])
+def test_cursor_tier_forced(cap):
+    # a small bitmap tier pushes most tail features into the cursor tier
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 4000)
+    run_and_check(rp, col, val, d, 0.4, 0, dense_features=cap)
+
+
+def test_multi_tile_lognormal():
+    rp, col, val, d = apss.workload("large", n=6000)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.5, 0)
+    assert info["cells"] == 6000 * 5999 // 2
+
+
+def test_split_invariance():
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 2500)
+    sets = []
+    for p in (0, 1, 300, 4096, d):
+        got, _ = apss.run(rp, col, val, d, 0.45, p)
+        g = apss.sort_pairs(got)
+        sets.append((g["i"].tobytes(), g["j"].tobytes()))
+    assert all(s == sets[0] for s in sets)
+
+
+def test_rescore_all_scores_exact():
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 2000)
+    got, info = apss.run(rp, col, val, d, 0.5, 0, rescore_all=1)
+    exp = oracle.apss_fp64(rp, col, val, d, 0.5)
+    g = apss.sort_pairs(got)
+    assert np.array_equal(g["i"], exp["i"]) and np.array_equal(g["j"], exp["j"])
+    assert np.array_equal(g["score"], exp["score"].astype(np.float32))
+
+
+def test_row_permutation():
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 2000)
+    n = len(rp) - 1
+    perm = np.random.default_rng(7).permutation(n)
+    lens = np.diff(rp)[perm]
+    rp2 = np.zeros(n + 1, np.int64)
+    rp2[1:] = np.cumsum(lens)
+    col2 = np.concatenate([col[rp[i]:rp[i + 1]] for i in perm])
+    val2 = np.concatenate([val[rp[i]:rp[i + 1]] for i in perm])
+    a, _ = apss.run(rp, col, val, d, 0.5, 0)
+    b, _ = apss.run(rp2, col2, val2, d, 0.5, 0)
+    inv = np.argsort(perm)  # new id -> old id is perm; old id of new row k is perm[k]
+    bi, bj = perm[b["i"]], perm[b["j"]]
+    lo, hi = np.minimum(bi, bj), np.maximum(bi, bj)
+    sb = set(zip(lo.tolist(), hi.tolist()))
+    sa = set(zip(a["i"].tolist(), a["j"].tolist()))
+    assert sa == sb and inv is not None
+
+
+def test_invalid_inputs_rejected():
+    rp, col, val, d = tiny_matrix([{1: 1.0, 3: 0.5}, {2: 1.0}], 5)
+    bad = col.copy()
+    bad[0], bad[1] = 3, 1  # descending inside a row
+    with pytest.raises(apss.ApssError) as e:
+        apss.run(rp, bad, val, d, 0.5, 0)
+    assert e.value.code == apss.APSS_E_INVALID
+    badv = val.copy()
+    badv[0] = np.nan
+    with pytest.raises(apss.ApssError) as e:
+        apss.run(rp, col, badv, d, 0.5, 0)
+    assert e.value.code == apss.APSS_E_INVALID
+    badc = col.copy()
+    badc[2] = 9
+    with pytest.raises(apss.ApssError) as e:
+        apss.run(rp, badc, val, d, 0.5, 0)
+    assert e.value.code == apss.APSS_E_INVALID
