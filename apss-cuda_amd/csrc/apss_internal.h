@@ -14,18 +14,24 @@ namespace apss {
 // ---------------------------------------------------------------------------
 // Layout constants
 //
-// Rows are cut into tiles of TILE consecutive rows.  A tile pair (I, J) owns a
-// TILE x TILE FP32 score tile in LDS (row stride TILE+1 so that the cells one
-// wave updates for a fixed column fall in different banks).
-constexpr int TILE = 128;
-constexpr int TSTRIDE = TILE + 1;
-constexpr int SCAN_THREADS = 1024;           // tile kernel workgroup (16 waves)
-constexpr int KD_MAX = 32768;                // features in the bitmap tier
+// Rows are cut into bands (I side, at most BAND_ROWS rows) and tiles (J side,
+// at most TILE_ROWS rows) of consecutive rows.  A band x tile pair owns a
+// BAND_ROWS x TILE_ROWS FP32 score tile in LDS (row stride TILE_ROWS+1 so the
+// cells one wave updates for a fixed I row fall in different banks).  Tile and
+// band heights are variable: a band/tile also ends before its bitmap-tier
+// entries exceed CAP_I / CAP_J, so its whole block always fits in LDS; since a
+// row holds at most KD_MAX <= CAP_J bitmap-tier entries, a single row fits.
+constexpr int BAND_ROWS = 128;
+constexpr int TILE_ROWS = 64;
+constexpr int TSTRIDE = TILE_ROWS + 1;
+constexpr int SCAN_THREADS = 1024;           // band kernel workgroup (16 waves)
+constexpr int KD_MAX = 4096;                 // features in the bitmap tier
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
-constexpr int AND_CHUNK_WORDS = 64;          // bitmap words intersected per pass
-constexpr int COMMON_CAP = AND_CHUNK_WORDS * 64;
-constexpr int S_LDS_CAP = 6144;              // cursor heads kept in LDS per band
-constexpr int S_LIST_MAX = 4096;             // longest list of the cursor tier
+constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
+constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
+constexpr int CAP_SI = 6144;                 // cursor-tier entries of a band kept in LDS
+constexpr int CAP_REC = 2048;                // common-feature records per pass
+constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
 constexpr int BIG_PRODUCTS = 32;             // common features with >= this many
                                              // products get a whole wave
 
@@ -61,31 +67,35 @@ struct Error {
 // ---------------------------------------------------------------------------
 // Device views
 
-// Row-tile structures of one matrix side (corpus or queries).
+// Tiled structures of one matrix side (corpus or queries) for one tiling.
 struct TileSide {
-  int64_t n = 0;               // rows
-  int32_t T = 0;               // tiles
-  const int64_t* rp = nullptr; // CSR (device)
+  int64_t n = 0;                  // rows
+  int32_t T = 0;                  // tiles (or bands)
+  const int64_t* tstart = nullptr;  // T+1 first row of each tile
+  const int64_t* rp = nullptr;    // CSR (device)
   const int32_t* col = nullptr;
   const float* val = nullptr;
-  const float* norm = nullptr;   // ||x_i||_2 rounded up
-  const float* hnorm = nullptr;  // ||x_i restricted to ids < p||_2 rounded up
-  // bitmap tier
-  const uint64_t* bmp = nullptr; // T * KW
-  const int32_t* ptr = nullptr;  // T * (KD+1), offsets inside the tile block
-  const int64_t* base = nullptr; // T+1, start of each tile block
-  const uint8_t* drow = nullptr; // local row of each bitmap-tier entry
+  const float* norm = nullptr;    // ||x_i||_2 rounded up
+  const float* hnorm = nullptr;   // ||x_i restricted to ids < p||_2 rounded up
+  // bitmap tier, per tile: presence bitmap, per-word prefix of present
+  // features, compact segment starts (present+1, relative to the tile block)
+  const uint64_t* bmp = nullptr;  // T * KW
+  const uint16_t* wpre = nullptr; // T * KW
+  const int64_t* cbase = nullptr; // T+1
+  const int32_t* cst = nullptr;
+  const int64_t* dbase = nullptr; // T+1, start of each tile block
+  const uint8_t* drow = nullptr;  // local row of each bitmap-tier entry
   const float* dval = nullptr;
   // cursor-tier entries of this side (one record per entry, grouped by tile)
-  const int64_t* sbase = nullptr;  // T+1
-  const int4* srec = nullptr;      // {local row, float bits of x, cursor, end}
+  const int64_t* sbase = nullptr; // T+1
+  const int4* srec = nullptr;     // {local row, float bits of x, first cursor, end}
 };
 
-// Cursor-tier inverted lists of the corpus side (ascending row per list).
+// Cursor-tier inverted lists of the corpus side: {row, float bits of value},
+// ascending row inside each list.
 struct CursorLists {
-  const int64_t* ptr = nullptr;  // d+1
-  const int32_t* row = nullptr;
-  const float* val = nullptr;
+  const int64_t* ptr = nullptr;   // d+1
+  const int2* ent = nullptr;
 };
 
 struct ScanArgs {
@@ -110,8 +120,8 @@ struct ScanArgs {
   int32_t band_count;
   int32_t pad2_;
   unsigned long long* stats; // [0] products, [1] head pairs, [2] cells
-  int32_t* snext_spill;      // per band: S_next beyond the LDS capacity
-  int64_t spill_stride;
+  int32_t* scur;             // cursor position of every cursor record (I side)
+  int32_t* snext_spill;      // next row of records beyond CAP_SI
 };
 
 // Arguments of the per-tile bitmap-tier build kernels.
@@ -119,22 +129,27 @@ struct BuildSide {
   int64_t n;
   int32_t T;
   int32_t KD;
+  int32_t KW;
   int32_t p;
   int32_t self;
-  int64_t row_off;            // global id of local row 0 (for cursor starts)
+  int32_t pad_;
+  const int64_t* tstart;      // T+1
   const int64_t* rp;
   const int32_t* col;
   const float* val;
   const int32_t* fmap;
-  int32_t* ptr;               // T*(KD+1)
+  int32_t* full;              // T*(KD+1) scratch: counts, then pointers
   uint64_t* bmp;              // T*KW
+  uint16_t* wpre;             // T*KW
+  int64_t* ctot;              // T+1 present features, then exclusive scan
+  int32_t* cst;
   int64_t* dtot;              // T+1 (counts, then exclusive scan)
   int64_t* stot;              // T+1
   uint8_t* drow;
   float* dval;
   int4* srec;
   const int64_t* sptr;        // corpus cursor lists
-  const int32_t* srow;
+  const int2* sent;
 };
 
 // ---------------------------------------------------------------------------

@@ -6,7 +6,7 @@
 
 namespace apss {
 
-// ---- kernels (defined in index_build.hip / tile_scan.hip / head_mfma.hip) ----
+// ---- kernels (defined in index_build.hip / tile_scan.hip) ----
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v);
 __global__ void k_copy_i64(int64_t* dst, const int64_t* src, int64_t n);
@@ -14,16 +14,17 @@ __global__ void k_validate_norms(int64_t n, int64_t d, int64_t nnz, const int64_
                                  const int32_t* col, const float* val, int32_t p, float* norm,
                                  float* hnorm, uint32_t* err);
 __global__ void k_col_hist(int64_t nnz, const int32_t* col, int32_t p, int64_t chunk, int32_t* cnt);
-__global__ void k_cursor_scatter(int64_t n, int64_t row_off, const int64_t* rp, const int32_t* col,
-                                 const float* val, const int32_t* fmap, unsigned long long* cur,
-                                 int32_t* srow, float* sval);
-__global__ void k_sort_small(const int32_t* lists, int32_t nl, const int64_t* sptr, int32_t* srow,
-                             float* sval);
-__global__ void k_sort_large(const int32_t* lists, const int64_t* sptr, int32_t* srow, float* sval);
+__global__ void k_cursor_scatter(int64_t n, const int64_t* rp, const int32_t* col, const float* val,
+                                 const int32_t* fmap, unsigned long long* cur, int2* sent);
+__global__ void k_sort_small(const int32_t* lists, int32_t nl, const int64_t* sptr, int2* sent);
+__global__ void k_sort_large(const int32_t* lists, const int64_t* sptr, int2* sent);
+__global__ void k_row_counts(int64_t n, const int64_t* rp, const int32_t* col, int32_t p,
+                             const int32_t* fmap, int32_t* dcnt, int32_t* scnt);
 __global__ void k_tile_count(BuildSide s);
 __global__ void k_scan_i64(int64_t* a, int64_t n);
-__global__ void k_tile_ptr_scan(int32_t* ptr, uint64_t* bmp, int32_t KD);
+__global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
+__global__ void k_tile_compact(BuildSide s);
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
@@ -69,11 +70,10 @@ struct Tiers {
   int64_t n_cursor_entries = 0;
   int32_t* fmap = nullptr;       // d
   int64_t* sptr = nullptr;       // d+1
-  int32_t* srow = nullptr;
-  float* sval = nullptr;
+  int2* sent = nullptr;          // {row, value bits}
 };
 
-// One matrix side: CSR on the device, norms and the per-tile structures.
+// One matrix side: CSR on the device and its norms.
 struct Side {
   int64_t n = 0, nnz = 0, d = 0;
   const int64_t* rp = nullptr;
@@ -83,17 +83,25 @@ struct Side {
   float* hnorm = nullptr;
   int32_t maxlen = 0;
   float maxnorm = 0.f;
+};
+
+// Per-tile structures of one tiling of a side.
+struct TileSet {
   int32_t T = 0;
-  int32_t* ptr = nullptr;
+  int64_t* tstart = nullptr;
   uint64_t* bmp = nullptr;
+  uint16_t* wpre = nullptr;
+  int64_t* cbase = nullptr;
+  int32_t* cst = nullptr;
   int64_t* dbase = nullptr;
-  int64_t* sbase = nullptr;
   uint8_t* drow = nullptr;
   float* dval = nullptr;
+  int64_t* sbase = nullptr;
   int4* srec = nullptr;
   int64_t nD = 0, nS = 0;
-  TileSide view() const;
 };
+
+TileSide make_view(const Side& s, const TileSet& ts);
 
 struct Timer {  // hipEvent pair on a stream
   hipEvent_t a = nullptr, b = nullptr;
@@ -124,11 +132,14 @@ struct ScanOutput {
 Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p);
 // Histogram + tier selection + cursor lists of a corpus side.
 Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap);
-// Per-tile bitmap tier + cursor records of a side, against corpus tiers.
-void build_tiles(Arena& ar, hipStream_t st, Side& s, const Tiers& tr, int32_t self);
+// Per-tile bitmap tier (+ cursor records when `records`) of a side against
+// corpus tiers, for tiles of at most max_rows rows and cap_d bitmap entries.
+TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
+                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records);
 // Band scan + rescore; output pairs stay on the device.
-ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side& J,
-                            const Tiers& tr, const RunConfig& rc);
+ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
+                            const Side& J, const TileSet& tiles, const Tiers& tr,
+                            const RunConfig& rc);
 
 // Full self-join into a result (host or device pairs per opts).
 int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);

@@ -95,22 +95,25 @@ static void fill_u64(hipStream_t st, unsigned long long* p, int64_t n, unsigned 
   APSS_HIP(hipGetLastError());
 }
 
-TileSide Side::view() const {
+TileSide make_view(const Side& s, const TileSet& ts) {
   TileSide v;
-  v.n = n;
-  v.T = T;
-  v.rp = rp;
-  v.col = col;
-  v.val = val;
-  v.norm = norm;
-  v.hnorm = hnorm;
-  v.bmp = bmp;
-  v.ptr = ptr;
-  v.base = dbase;
-  v.drow = drow;
-  v.dval = dval;
-  v.sbase = sbase;
-  v.srec = srec;
+  v.n = s.n;
+  v.T = ts.T;
+  v.tstart = ts.tstart;
+  v.rp = s.rp;
+  v.col = s.col;
+  v.val = s.val;
+  v.norm = s.norm;
+  v.hnorm = s.hnorm;
+  v.bmp = ts.bmp;
+  v.wpre = ts.wpre;
+  v.cbase = ts.cbase;
+  v.cst = ts.cst;
+  v.dbase = ts.dbase;
+  v.drow = ts.drow;
+  v.dval = ts.dval;
+  v.sbase = ts.sbase;
+  v.srec = ts.srec;
   return v;
 }
 
@@ -189,7 +192,6 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
     s.maxlen = (int32_t)ml;
     s.maxnorm = mn;
   }
-  s.T = (int32_t)((s.n + TILE - 1) / TILE);
   return s;
 }
 
@@ -215,7 +217,9 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   ar.release(cnt);
 
   const int32_t need = self ? 2 : 1;
-  const int64_t T = std::max<int64_t>(1, (s.n + TILE - 1) / TILE);
+  // bitmap tier: the lists present in a sizeable share of the 64-row tiles,
+  // plus every list too long for the cursor tier
+  const int64_t T = std::max<int64_t>(1, (s.n + TILE_ROWS - 1) / TILE_ROWS);
   const int64_t lmin = std::max<int64_t>(need, T / 8);
   int32_t cap = kd_cap > 0 ? std::min(kd_cap, KD_MAX) : KD_MAX;
   std::vector<int32_t> cand;
@@ -230,16 +234,17 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
                                         std::to_string(cap)};
   std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) { return h[x] > h[y]; });
   if ((int64_t)cand.size() > cap) cand.resize(cap);
-  // bitmap-tier features ordered by id (keeps the tile blocks in id order)
-  std::sort(cand.begin(), cand.end());
+  tr.n_bitmap = (int32_t)cand.size();
+  tr.KD = std::max(64, (tr.n_bitmap + 63) / 64 * 64);
+  tr.KW = tr.KD / 64;
+  // Frequency rank k goes to word k % KW, bit k / KW: every bitmap word mixes
+  // frequent and rare features, so the waves that own words get equal work.
   std::vector<int32_t> fm((size_t)d, F_IGNORE);
   for (int64_t c = 0; c < std::min<int64_t>(p, d); ++c) fm[c] = F_HEAD;
   for (int64_t c = p; c < d; ++c)
     if (h[c] >= need) fm[c] = F_CURSOR;
-  for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
-  tr.n_bitmap = (int32_t)cand.size();
-  tr.KD = std::max(64, (tr.n_bitmap + 63) / 64 * 64);
-  tr.KW = tr.KD / 64;
+  for (size_t k = 0; k < cand.size(); ++k)
+    fm[cand[k]] = (int32_t)((k % tr.KW) * 64 + k / tr.KW);
   std::vector<int64_t> sp((size_t)d + 1, 0);
   std::vector<int32_t> small_lists, large_lists;
   for (int64_t c = 0; c < d; ++c) {
@@ -253,8 +258,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
     throw Error{APSS_E_UNSUPPORTED, "cursor tier larger than 2^31 entries"};
   tr.fmap = ar.alloc<int32_t>(d);
   tr.sptr = ar.alloc<int64_t>(d + 1);
-  tr.srow = ar.alloc<int32_t>(tr.n_cursor_entries);
-  tr.sval = ar.alloc<float>(tr.n_cursor_entries);
+  tr.sent = ar.alloc<int2>(tr.n_cursor_entries);
   APSS_HIP(hipMemcpyAsync(tr.fmap, fm.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
   APSS_HIP(hipMemcpyAsync(tr.sptr, sp.data(), sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
   if (tr.n_cursor_entries > 0) {
@@ -262,15 +266,14 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
     hipLaunchKernelGGL(k_copy_i64, dim3(blocks_for(d + 1, 256, 4096)), dim3(256), 0, st,
                        (int64_t*)cur, tr.sptr, d + 1);
     hipLaunchKernelGGL(k_cursor_scatter, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n,
-                       (int64_t)0, s.rp, s.col, s.val, tr.fmap, cur, tr.srow, tr.sval);
+                       s.rp, s.col, s.val, tr.fmap, cur, tr.sent);
     APSS_HIP(hipGetLastError());
     int32_t* lists = ar.alloc<int32_t>((int64_t)(small_lists.size() + large_lists.size()));
     if (!small_lists.empty()) {
       APSS_HIP(hipMemcpyAsync(lists, small_lists.data(), sizeof(int32_t) * small_lists.size(),
                               hipMemcpyHostToDevice, st));
       hipLaunchKernelGGL(k_sort_small, dim3(blocks_for((int64_t)small_lists.size(), 4, 1 << 30)),
-                         dim3(256), 0, st, lists, (int32_t)small_lists.size(), tr.sptr, tr.srow,
-                         tr.sval);
+                         dim3(256), 0, st, lists, (int32_t)small_lists.size(), tr.sptr, tr.sent);
       APSS_HIP(hipGetLastError());
     }
     if (!large_lists.empty()) {
@@ -278,7 +281,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
       APSS_HIP(hipMemcpyAsync(ll, large_lists.data(), sizeof(int32_t) * large_lists.size(),
                               hipMemcpyHostToDevice, st));
       hipLaunchKernelGGL(k_sort_large, dim3((unsigned)large_lists.size()), dim3(256), 0, st, ll,
-                         tr.sptr, tr.srow, tr.sval);
+                         tr.sptr, tr.sent);
       APSS_HIP(hipGetLastError());
     }
     APSS_HIP(hipStreamSynchronize(st));
@@ -291,65 +294,120 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 // ---------------------------------------------------------------------------
 // build_tiles
 
-void build_tiles(Arena& ar, hipStream_t st, Side& s, const Tiers& tr, int32_t self) {
-  const int32_t T = s.T;
-  s.ptr = ar.alloc<int32_t>((int64_t)T * (tr.KD + 1));
-  s.bmp = ar.alloc<uint64_t>((int64_t)T * tr.KW);
-  s.dbase = ar.alloc<int64_t>((int64_t)T + 1);
-  s.sbase = ar.alloc<int64_t>((int64_t)T + 1);
-  fill_i32(st, s.ptr, (int64_t)T * (tr.KD + 1), 0);
+// Greedy cut of the rows into tiles of at most max_rows rows whose bitmap-tier
+// entries (padded to a multiple of 4) stay within cap_d and whose cursor-tier
+// entries stay within cap_s (a single row always forms a tile).
+static std::vector<int64_t> cut_tiles(const std::vector<int32_t>& dc, const std::vector<int32_t>& sc,
+                                      int32_t max_rows, int32_t cap_d, int32_t cap_s) {
+  std::vector<int64_t> ts;
+  const int64_t n = (int64_t)dc.size();
+  ts.push_back(0);
+  int64_t rows = 0, sd = 0, ss = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    const int64_t nd = ((sd + dc[r] + 3) / 4) * 4;
+    if (rows > 0 && (rows == max_rows || nd > cap_d || ss + sc[r] > cap_s)) {
+      ts.push_back(r);
+      rows = sd = ss = 0;
+    }
+    ++rows;
+    sd += dc[r];
+    ss += sc[r];
+  }
+  if (n > 0) ts.push_back(n);
+  return ts;
+}
+
+TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
+                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records) {
+  TileSet ts;
+  std::vector<int64_t> cuts(1, 0);
+  if (s.n > 0) {
+    int32_t* cnts = ar.alloc<int32_t>(2 * s.n);
+    hipLaunchKernelGGL(k_row_counts, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n, s.rp,
+                       s.col, tr.p, tr.fmap, cnts, cnts + s.n);
+    APSS_HIP(hipGetLastError());
+    std::vector<int32_t> dc((size_t)s.n), sc((size_t)s.n);
+    APSS_HIP(hipMemcpyAsync(dc.data(), cnts, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipMemcpyAsync(sc.data(), cnts + s.n, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    ar.release(cnts);
+    if (!records) std::fill(sc.begin(), sc.end(), 0);
+    cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s);
+  }
+  const int32_t T = (int32_t)cuts.size() - 1;
+  ts.T = T;
+  ts.tstart = ar.alloc<int64_t>(T + 1);
+  APSS_HIP(hipMemcpyAsync(ts.tstart, cuts.data(), sizeof(int64_t) * (T + 1), hipMemcpyHostToDevice, st));
+  ts.bmp = ar.alloc<uint64_t>((int64_t)T * tr.KW);
+  ts.wpre = ar.alloc<uint16_t>((int64_t)T * tr.KW);
+  ts.cbase = ar.alloc<int64_t>((int64_t)T + 1);
+  ts.dbase = ar.alloc<int64_t>((int64_t)T + 1);
+  ts.sbase = ar.alloc<int64_t>((int64_t)T + 1);
+  int32_t* full = ar.alloc<int32_t>((int64_t)T * (tr.KD + 1));
+  fill_i32(st, full, (int64_t)T * (tr.KD + 1), 0);
   BuildSide b{};
   b.n = s.n;
   b.T = T;
   b.KD = tr.KD;
+  b.KW = tr.KW;
   b.p = tr.p;
   b.self = self;
-  b.row_off = 0;
+  b.tstart = ts.tstart;
   b.rp = s.rp;
   b.col = s.col;
   b.val = s.val;
   b.fmap = tr.fmap;
-  b.ptr = s.ptr;
-  b.bmp = s.bmp;
-  b.dtot = s.dbase;
-  b.stot = s.sbase;
+  b.full = full;
+  b.bmp = ts.bmp;
+  b.wpre = ts.wpre;
+  b.ctot = ts.cbase;
+  b.dtot = ts.dbase;
+  b.stot = ts.sbase;
   b.sptr = tr.sptr;
-  b.srow = tr.srow;
+  b.sent = tr.sent;
   if (T > 0) {
     hipLaunchKernelGGL(k_tile_count, dim3(T), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, s.dbase, (int64_t)T);
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, s.sbase, (int64_t)T);
-    hipLaunchKernelGGL(k_tile_ptr_scan, dim3(T), dim3(1024), 0, st, s.ptr, s.bmp, tr.KD);
+    hipLaunchKernelGGL(k_tile_ptr_scan, dim3(T), dim3(1024), 0, st, b);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, ts.dbase, (int64_t)T);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, ts.sbase, (int64_t)T);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, ts.cbase, (int64_t)T);
     APSS_HIP(hipGetLastError());
   } else {
-    fill_u64(st, (unsigned long long*)s.dbase, 1, 0);
-    fill_u64(st, (unsigned long long*)s.sbase, 1, 0);
+    fill_u64(st, (unsigned long long*)ts.dbase, 1, 0);
+    fill_u64(st, (unsigned long long*)ts.sbase, 1, 0);
+    fill_u64(st, (unsigned long long*)ts.cbase, 1, 0);
   }
-  int64_t tot[2] = {0, 0};
-  APSS_HIP(hipMemcpyAsync(&tot[0], s.dbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  APSS_HIP(hipMemcpyAsync(&tot[1], s.sbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  int64_t tot[3] = {0, 0, 0};
+  APSS_HIP(hipMemcpyAsync(&tot[0], ts.dbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipMemcpyAsync(&tot[1], ts.sbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipMemcpyAsync(&tot[2], ts.cbase + T, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   APSS_HIP(hipStreamSynchronize(st));
-  s.nD = tot[0];
-  s.nS = tot[1];
-  if (s.nD >= ((int64_t)1 << 31))
-    throw Error{APSS_E_UNSUPPORTED, "bitmap tier larger than 2^31 entries"};
-  s.drow = ar.alloc<uint8_t>(s.nD);
-  s.dval = ar.alloc<float>(s.nD);
-  s.srec = ar.alloc<int4>(s.nS);
-  b.drow = s.drow;
-  b.dval = s.dval;
-  b.srec = s.srec;
+  ts.nD = tot[0];
+  ts.nS = records ? tot[1] : 0;
+  ts.drow = ar.alloc<uint8_t>(ts.nD);
+  ts.dval = ar.alloc<float>(ts.nD);
+  ts.cst = ar.alloc<int32_t>(tot[2]);
+  ts.srec = ar.alloc<int4>(records ? tot[1] : 0);
+  b.drow = ts.drow;
+  b.dval = ts.dval;
+  b.cst = ts.cst;
+  b.srec = records ? ts.srec : nullptr;
   if (T > 0) {
     hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_tile_compact, dim3(T), dim3(256), 0, st, b);
     APSS_HIP(hipGetLastError());
   }
+  APSS_HIP(hipStreamSynchronize(st));
+  ar.release(full);
+  return ts;
 }
 
 // ---------------------------------------------------------------------------
 // scan + rescore
 
-ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side& J,
-                            const Tiers& tr, const RunConfig& rc) {
+ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
+                            const Side& J, const TileSet& tiles, const Tiers& tr,
+                            const RunConfig& rc) {
   ScanOutput so;
   const int64_t maxlen = std::max(I.maxlen, J.maxlen);
   const double maxn = (double)std::max(I.maxnorm, 1e-30f) * (double)std::max(J.maxnorm, 1e-30f);
@@ -365,18 +423,18 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side
   cap = std::max<int64_t>(cap, 1 << 16);
   int64_t bcap = std::max<int64_t>(1 << 20, cap / 64);
   unsigned long long* ctr = ar.alloc<unsigned long long>(8);
-  int32_t* spill = ar.alloc<int32_t>(2 * std::max<int64_t>(I.nS, 1));
+  int32_t* scur = ar.alloc<int32_t>(bands.nS);
+  int32_t* spill = ar.alloc<int32_t>(bands.nS);
   Timer tm(st);
   for (int attempt = 0; attempt < 3; ++attempt) {
     apss_pair* out = ar.alloc<apss_pair>(cap);
     unsigned long long* band = ar.alloc<unsigned long long>(bcap);
     fill_u64(st, ctr, 8, 0);
     ScanArgs a{};
-    a.I = I.view();
-    a.J = J.view();
+    a.I = make_view(I, bands);
+    a.J = make_view(J, tiles);
     a.S.ptr = tr.sptr;
-    a.S.row = tr.srow;
-    a.S.val = tr.sval;
+    a.S.ent = tr.sent;
     a.KD = tr.KD;
     a.KW = tr.KW;
     a.p = tr.p;
@@ -394,12 +452,13 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Side
     a.band_cap = bcap;
     a.rescore_all = rc.rescore_all;
     a.band_first = 0;
-    a.band_count = I.T;
+    a.band_count = bands.T;
     a.stats = ctr + 2;
+    a.scur = scur;
     a.snext_spill = spill;
     tm.start();
-    if (I.T > 0 && J.T > 0) {
-      hipLaunchKernelGGL(k_band_scan, dim3(I.T), dim3(SCAN_THREADS), 0, st, a);
+    if (bands.T > 0 && tiles.T > 0) {
+      hipLaunchKernelGGL(k_band_scan, dim3(bands.T), dim3(SCAN_THREADS), 0, st, a);
       APSS_HIP(hipGetLastError());
     }
     so.ms_scan = tm.stop_ms();
@@ -484,14 +543,15 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->ms_upload = tm.stop_ms();
   tm.start();
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
-  build_tiles(ar, st, s, tr, 1);
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_SI, true);
+  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false);
   out->ms_index = tm.stop_ms();
   RunConfig rc;
   rc.t = t;
   rc.p = p;
   rc.self = 1;
   rc.rescore_all = o->rescore_all;
-  ScanOutput so = scan_and_rescore(ar, st, s, s, tr, rc);
+  ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   out->ms_scan = so.ms_scan;
   out->ms_rescore = so.ms_rescore;
   out->count = so.count;

@@ -123,14 +123,12 @@ __global__ __launch_bounds__(1024) void k_col_hist(int64_t nnz, const int32_t* _
 // ---------------------------------------------------------------------------
 // cursor tier: scatter into one list per feature, then sort each list by row
 
-__global__ __launch_bounds__(256) void k_cursor_scatter(int64_t n, int64_t row_off,
-                                                        const int64_t* __restrict__ rp,
+__global__ __launch_bounds__(256) void k_cursor_scatter(int64_t n, const int64_t* __restrict__ rp,
                                                         const int32_t* __restrict__ col,
                                                         const float* __restrict__ val,
                                                         const int32_t* __restrict__ fmap,
                                                         unsigned long long* __restrict__ cur,
-                                                        int32_t* __restrict__ srow,
-                                                        float* __restrict__ sval) {
+                                                        int2* __restrict__ sent) {
   const int lane = lane_id();
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
@@ -138,8 +136,7 @@ __global__ __launch_bounds__(256) void k_cursor_scatter(int64_t n, int64_t row_o
       int32_t c = col[e];
       if (fmap[c] != F_CURSOR) continue;
       unsigned long long q = atomicAdd(&cur[c], 1ull);
-      srow[q] = (int32_t)(r + row_off);
-      sval[q] = val[e];
+      sent[q] = make_int2((int32_t)r, __float_as_int(val[e]));
     }
   }
 }
@@ -147,46 +144,38 @@ __global__ __launch_bounds__(256) void k_cursor_scatter(int64_t n, int64_t row_o
 // One wave sorts one list of length <= 64 (bitonic network over lanes).
 __global__ __launch_bounds__(256) void k_sort_small(const int32_t* __restrict__ lists, int32_t nl,
                                                     const int64_t* __restrict__ sptr,
-                                                    int32_t* __restrict__ srow,
-                                                    float* __restrict__ sval) {
+                                                    int2* __restrict__ sent) {
   int w = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   if (w >= nl) return;
   const int lane = lane_id();
   int32_t f = lists[w];
   int64_t a = sptr[f];
   int len = (int)(sptr[f + 1] - a);
-  int32_t key = lane < len ? srow[a + lane] : INT32_MAX;
-  float v = lane < len ? sval[a + lane] : 0.f;
+  int2 e = lane < len ? sent[a + lane] : make_int2(INT32_MAX, 0);
   for (int k = 2; k <= 64; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      int32_t ok = __shfl_xor(key, j);
-      float ov = __shfl_xor(v, j);
+      int ox = __shfl_xor(e.x, j), oy = __shfl_xor(e.y, j);
       bool up = ((lane & k) == 0);
       bool lower = ((lane & j) == 0);
-      bool take_min = (lower == up);
-      bool swap = take_min ? (ok < key) : (ok > key);
-      if (swap) { key = ok; v = ov; }
+      bool swap = (lower == up) ? (ox < e.x) : (ox > e.x);
+      if (swap) { e.x = ox; e.y = oy; }
     }
   }
-  if (lane < len) { srow[a + lane] = key; sval[a + lane] = v; }
+  if (lane < len) sent[a + lane] = e;
 }
 
 // One workgroup sorts one list of length <= S_LIST_MAX in LDS.
 __global__ __launch_bounds__(256) void k_sort_large(const int32_t* __restrict__ lists,
                                                     const int64_t* __restrict__ sptr,
-                                                    int32_t* __restrict__ srow,
-                                                    float* __restrict__ sval) {
-  __shared__ int32_t key[S_LIST_MAX];
-  __shared__ float v[S_LIST_MAX];
+                                                    int2* __restrict__ sent) {
+  __shared__ int2 buf[S_LIST_MAX];
   int32_t f = lists[blockIdx.x];
   int64_t a = sptr[f];
   int len = (int)(sptr[f + 1] - a);
   int np = 64;
   while (np < len) np <<= 1;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) {
-    key[i] = i < len ? srow[a + i] : INT32_MAX;
-    v[i] = i < len ? sval[a + i] : 0.f;
-  }
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    buf[i] = i < len ? sent[a + i] : make_int2(INT32_MAX, 0);
   __syncthreads();
   for (int k = 2; k <= np; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -194,33 +183,55 @@ __global__ __launch_bounds__(256) void k_sort_large(const int32_t* __restrict__ 
         int ixj = i ^ j;
         if (ixj > i) {
           bool up = ((i & k) == 0);
-          int32_t ki = key[i], kj = key[ixj];
-          if ((ki > kj) == up) {
-            key[i] = kj; key[ixj] = ki;
-            float t = v[i]; v[i] = v[ixj]; v[ixj] = t;
-          }
+          int2 x = buf[i], y = buf[ixj];
+          if ((x.x > y.x) == up) { buf[i] = y; buf[ixj] = x; }
         }
       }
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < len; i += blockDim.x) { srow[a + i] = key[i]; sval[a + i] = v[i]; }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) sent[a + i] = buf[i];
+}
+
+// ---------------------------------------------------------------------------
+// per-row tier counts (inputs of the host tiling)
+
+__global__ __launch_bounds__(256) void k_row_counts(int64_t n, const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ col, int32_t p,
+                                                    const int32_t* __restrict__ fmap,
+                                                    int32_t* __restrict__ dcnt,
+                                                    int32_t* __restrict__ scnt) {
+  const int lane = lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
+    int dc = 0, sc = 0;
+    for (int64_t e = rp[r] + lane; e < rp[r + 1]; e += 64) {
+      int32_t c = col[e];
+      if (c < p) continue;
+      int32_t f = fmap[c];
+      dc += f >= 0;
+      sc += f == F_CURSOR;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      dc += __shfl_xor(dc, o);
+      sc += __shfl_xor(sc, o);
+    }
+    if (lane == 0) { dcnt[r] = dc; scnt[r] = sc; }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // per-tile bitmap tier
 
-
-
 // Count bitmap-tier entries per (tile, feature) into slot f+1 of the tile's
-// pointer row and the tile totals of both tiers.
+// scratch pointer row, and the tile totals of both tiers.
 __global__ __launch_bounds__(256) void k_tile_count(BuildSide s) {
   __shared__ unsigned long long dcount, scount;
   const int tile = blockIdx.x;
   if (threadIdx.x == 0) { dcount = 0; scount = 0; }
   __syncthreads();
-  int64_t r0 = (int64_t)tile * TILE, r1 = r0 + TILE < s.n ? r0 + TILE : s.n;
-  int32_t* prow = s.ptr + (int64_t)tile * (s.KD + 1);
+  int64_t r0 = s.tstart[tile], r1 = s.tstart[tile + 1];
+  int32_t* prow = s.full + (int64_t)tile * (s.KD + 1);
   unsigned long long dc = 0, sc = 0;
   for (int64_t e = s.rp[r0] + threadIdx.x; e < s.rp[r1]; e += blockDim.x) {
     int32_t c = s.col[e];
@@ -232,7 +243,8 @@ __global__ __launch_bounds__(256) void k_tile_count(BuildSide s) {
   atomicAdd(&dcount, dc);
   atomicAdd(&scount, sc);
   __syncthreads();
-  if (threadIdx.x == 0) { s.dtot[tile] = (int64_t)dcount; s.stot[tile] = (int64_t)scount; }
+  // blocks are padded to multiples of 4 entries (32/128-bit staging copies)
+  if (threadIdx.x == 0) { s.dtot[tile] = (int64_t)((dcount + 3) & ~3ull); s.stot[tile] = (int64_t)scount; }
 }
 
 // Exclusive scan of a[0..n) in place, a[n] = total (single workgroup).
@@ -261,21 +273,26 @@ __global__ __launch_bounds__(1024) void k_scan_i64(int64_t* a, int64_t n) {
 }
 
 // Per tile: counts c[f] (slot f+1) -> exclusive starts E[f] (slot f+1), slot 0
-// = 0, plus the presence bitmap.  After the scatter, which advances slot f+1
-// from E[f] to E[f]+c[f] = E[f+1], the row is the standard pointer array.
-__global__ __launch_bounds__(1024) void k_tile_ptr_scan(int32_t* ptr, uint64_t* bmp, int32_t KD) {
+// = 0, plus the presence bitmap and the number of present features.  After
+// the scatter, which advances slot f+1 from E[f] to E[f]+c[f] = E[f+1], the
+// row is the standard pointer array.
+__global__ __launch_bounds__(1024) void k_tile_ptr_scan(BuildSide s) {
   __shared__ int32_t part[1024];
-  __shared__ int32_t carry;
+  __shared__ int32_t carry, present;
   const int tile = blockIdx.x;
-  int32_t* prow = ptr + (int64_t)tile * (KD + 1);
-  uint64_t* brow = bmp + (int64_t)tile * (KD / 64);
-  if (threadIdx.x == 0) { carry = 0; prow[0] = 0; }
+  const int32_t KD = s.KD;
+  int32_t* prow = s.full + (int64_t)tile * (KD + 1);
+  uint64_t* brow = s.bmp + (int64_t)tile * s.KW;
+  if (threadIdx.x == 0) { carry = 0; present = 0; prow[0] = 0; }
   __syncthreads();
   for (int32_t base = 0; base < KD; base += 1024) {
     int32_t f = base + threadIdx.x;
     int32_t x = f < KD ? prow[f + 1] : 0;
     unsigned long long m = __ballot(x > 0);
-    if (lane_id() == 0 && f < KD) brow[f / 64] = m;
+    if (lane_id() == 0 && f < KD) {
+      brow[f / 64] = m;
+      atomicAdd(&present, __popcll(m));
+    }
     part[threadIdx.x] = x;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -289,13 +306,14 @@ __global__ __launch_bounds__(1024) void k_tile_ptr_scan(int32_t* ptr, uint64_t* 
     if (threadIdx.x == 1023) carry += part[1023];
     __syncthreads();
   }
+  if (threadIdx.x == 0) s.ctot[tile] = present + 1;  // starts + end sentinel
 }
 
-__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo, int64_t hi,
+__device__ __forceinline__ int64_t lower_bound_row(const int2* a, int64_t lo, int64_t hi,
                                                    int32_t key) {
   while (lo < hi) {
     int64_t m = (lo + hi) >> 1;
-    if (a[m] < key) lo = m + 1; else hi = m;
+    if (a[m].x < key) lo = m + 1; else hi = m;
   }
   return lo;
 }
@@ -303,16 +321,16 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo,
 // Scatter bitmap-tier entries into their tile blocks and write the cursor
 // records of this side's cursor-tier entries.
 __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
-  __shared__ int32_t rps[TILE + 1];
+  __shared__ int32_t rps[BAND_ROWS + 1];
   __shared__ unsigned int sfill;
   const int tile = blockIdx.x;
-  int64_t r0 = (int64_t)tile * TILE, r1 = r0 + TILE < s.n ? r0 + TILE : s.n;
+  int64_t r0 = s.tstart[tile], r1 = s.tstart[tile + 1];
   int nr = (int)(r1 - r0);
   int64_t e0 = s.rp[r0];
   for (int i = threadIdx.x; i <= nr; i += blockDim.x) rps[i] = (int32_t)(s.rp[r0 + i] - e0);
   if (threadIdx.x == 0) sfill = 0;
   __syncthreads();
-  int32_t* prow = s.ptr + (int64_t)tile * (s.KD + 1);
+  int32_t* prow = s.full + (int64_t)tile * (s.KD + 1);
   int64_t dbase = s.dtot[tile], sbase = s.stot[tile];
   int32_t ne = rps[nr];
   for (int32_t k = threadIdx.x; k < ne; k += blockDim.x) {
@@ -321,8 +339,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
     if (c < s.p) continue;
     int32_t f = s.fmap[c];
     if (f < 0 && f != F_CURSOR) continue;
-    // local row of entry k: last r with rps[r] <= k
-    int lo = 0, hi = nr;
+    int lo = 0, hi = nr;  // local row of entry k: last r with rps[r] <= k
     while (hi - lo > 1) {
       int m = (lo + hi) >> 1;
       if (rps[m] <= k) lo = m; else hi = m;
@@ -331,13 +348,40 @@ __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
       int32_t q = atomicAdd(&prow[f + 1], 1);
       s.drow[dbase + q] = (uint8_t)lo;
       s.dval[dbase + q] = s.val[e];
-    } else {
+    } else if (s.srec) {  // cursor records only for the band (I) side
       unsigned int q = atomicAdd(&sfill, 1u);
       int64_t a = s.sptr[c], b = s.sptr[c + 1];
-      int64_t start = s.self ? lower_bound_i32(s.srow, a, b, (int32_t)(s.row_off + r0 + lo + 1)) : a;
+      int64_t start = s.self ? lower_bound_row(s.sent, a, b, (int32_t)(r0 + lo + 1)) : a;
       s.srec[sbase + q] = make_int4(lo, __float_as_int(s.val[e]), (int)start, (int)b);
     }
   }
+}
+
+// Compact the pointer row of each tile: segment starts of the present
+// features (in id order) and the per-word prefix counts of the bitmap.
+__global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {
+  __shared__ int32_t wp[KW_MAX + 1];
+  const int tile = blockIdx.x;
+  const uint64_t* brow = s.bmp + (int64_t)tile * s.KW;
+  const int32_t* prow = s.full + (int64_t)tile * (s.KD + 1);
+  if (threadIdx.x == 0) {
+    int32_t acc = 0;
+    for (int w = 0; w < s.KW; ++w) { wp[w] = acc; acc += __popcll(brow[w]); }
+    wp[s.KW] = acc;
+  }
+  __syncthreads();
+  uint16_t* wrow = s.wpre + (int64_t)tile * s.KW;
+  int32_t* crow = s.cst + s.ctot[tile];
+  for (int w = threadIdx.x; w < s.KW; w += blockDim.x) wrow[w] = (uint16_t)wp[w];
+  for (int32_t f = threadIdx.x; f < s.KD; f += blockDim.x) {
+    uint64_t m = brow[f / 64];
+    int b = f % 64;
+    if ((m >> b) & 1ull) {
+      int rank = wp[f / 64] + __popcll(m & ((1ull << b) - 1ull));
+      crow[rank] = prow[f];
+    }
+  }
+  if (threadIdx.x == 0) crow[wp[s.KW]] = prow[s.KD];
 }
 
 }  // namespace apss

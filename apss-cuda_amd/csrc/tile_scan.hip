@@ -31,17 +31,6 @@
 
 namespace apss {
 
-constexpr int GROUP_FEATURES = 1024;     // bitmap features intersected per pass
-constexpr int N_GROUPS = KD_MAX / GROUP_FEATURES;
-constexpr int LIST_CAP = GROUP_FEATURES; // records per list and pass
-constexpr int S_CAP = 4096;              // cursor heads kept in LDS
-
-struct Rec {  // one common bitmap-tier feature of a tile pair
-  int32_t a;  // first I entry
-  int32_t b;  // first J entry
-  int32_t ab; // ca | cb << 16
-};
-
 __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
                                           const float* __restrict__ av, int64_t an,
                                           const int32_t* __restrict__ bc,
@@ -59,61 +48,98 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
   return s;
 }
 
+// Copy one tile block (bitmap words, word prefixes, compact starts, entries)
+// from HBM into LDS.  Block sizes are padded to multiples of 4 entries so the
+// entries move as 32-bit / 128-bit words.
+__device__ __forceinline__ void stage_block(const TileSide& S, int32_t t, int KW, uint64_t* bmp,
+                                            uint16_t* wpre, uint16_t* cst, uint8_t* rows,
+                                            float* vals, int tid) {
+  for (int w = tid; w < KW; w += SCAN_THREADS) {
+    bmp[w] = S.bmp[(int64_t)t * KW + w];
+    wpre[w] = S.wpre[(int64_t)t * KW + w];
+  }
+  const int64_t c0 = S.cbase[t];
+  const int ncs = (int)(S.cbase[t + 1] - c0);  // present features + 1
+  for (int k = tid; k < ncs; k += SCAN_THREADS) cst[k] = (uint16_t)S.cst[c0 + k];
+  const int64_t d0 = S.dbase[t];
+  const int nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
+  const uint32_t* r4 = reinterpret_cast<const uint32_t*>(S.drow + d0);
+  const float4* v4 = reinterpret_cast<const float4*>(S.dval + d0);
+  for (int k = tid; k < nd4; k += SCAN_THREADS) {
+    reinterpret_cast<uint32_t*>(rows)[k] = r4[k];
+    reinterpret_cast<float4*>(vals)[k] = v4[k];
+  }
+}
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
-  __shared__ float tile[TILE * TSTRIDE];
-  __shared__ uint64_t bmpI[KW_MAX];
-  __shared__ Rec big[LIST_CAP];
-  __shared__ Rec small[LIST_CAP];
-  __shared__ int32_t s_cur[S_CAP];
-  __shared__ int32_t s_nxt[S_CAP];
-  __shared__ float normI[TILE], normJ[TILE], hnI[TILE], hnJ[TILE];
-  __shared__ int32_t lenI[TILE], lenJ[TILE];
-  __shared__ unsigned int n_big[N_GROUPS], n_small[N_GROUPS];  // per pass, zeroed per tile pair
+  __shared__ float tile[BAND_ROWS * TSTRIDE];
+  __shared__ uint64_t bI[KW_MAX], bJ[KW_MAX];
+  __shared__ uint16_t wpI[KW_MAX], wpJ[KW_MAX];
+  __shared__ uint16_t cI[KD_MAX + 1], cJ[KD_MAX + 1];
+  __shared__ __align__(16) uint8_t rI[CAP_I];
+  __shared__ __align__(16) uint8_t rJ[CAP_J];
+  __shared__ __align__(16) float vI[CAP_I];
+  __shared__ __align__(16) float vJ[CAP_J];
+  __shared__ int32_t s_nxt[CAP_SI];
+  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
+  __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
+  __shared__ int32_t J_first_s;
   __shared__ unsigned long long acc_products, acc_head, acc_cells;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid / 64;
   constexpr int NW = SCAN_THREADS / 64;
+  const int KW = a.KW;
 
   const int32_t I = a.band_first + blockIdx.x;
   if (I >= a.I.T) return;
-  const int64_t i0 = (int64_t)I * TILE;
-  const int nI = (int)((a.I.n - i0) < TILE ? (a.I.n - i0) : TILE);
+  const int64_t i0 = a.I.tstart[I];
+  const int nI = (int)(a.I.tstart[I + 1] - i0);
 
   // ---- band set-up -------------------------------------------------------
-  for (int c = tid; c < TILE * TSTRIDE; c += SCAN_THREADS) tile[c] = 0.f;
-  for (int w = tid; w < a.KW; w += SCAN_THREADS) bmpI[w] = a.I.bmp[(int64_t)I * a.KW + w];
-  if (tid < TILE) {
+  stage_block(a.I, I, KW, bI, wpI, cI, rI, vI, tid);
+  for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0.f;
+  if (tid < BAND_ROWS) {
     bool ok = tid < nI;
     int64_t r = i0 + tid;
     normI[tid] = ok ? a.I.norm[r] : 0.f;
     hnI[tid] = ok ? a.I.hnorm[r] : 0.f;
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
   }
-  if (tid == 0) { acc_products = 0; acc_head = 0; acc_cells = 0; }
-  if (tid < N_GROUPS) { n_big[tid] = 0; n_small[tid] = 0; }
+  if (tid == 0) {
+    acc_products = 0; acc_head = 0; acc_cells = 0;
+    int32_t jf = 0;
+    if (a.self) {  // first tile whose rows end after the band's first row
+      int32_t lo = 0, hi = a.J.T;
+      while (lo < hi) {
+        int32_t m = (lo + hi) >> 1;
+        if (a.J.tstart[m + 1] <= i0) lo = m + 1; else hi = m;
+      }
+      jf = lo;
+    }
+    J_first_s = jf;
+  }
   const int64_t sb = a.I.sbase[I];
   const int nS = (int)(a.I.sbase[I + 1] - sb);
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     int4 r = a.I.srec[sb + k];
-    int32_t nx = r.z < r.w ? a.S.row[r.z] : INT32_MAX;
-    if (k < S_CAP) { s_cur[k] = r.z; s_nxt[k] = nx; }
-    else { a.snext_spill[2 * (sb + k)] = r.z; a.snext_spill[2 * (sb + k) + 1] = nx; }
+    int32_t nx = r.z < r.w ? a.S.ent[r.z].x : INT32_MAX;
+    a.scur[sb + k] = r.z;
+    if (k < CAP_SI) s_nxt[k] = nx; else a.snext_spill[sb + k] = nx;
   }
-  const uint64_t* bmpJall = a.J.bmp;
-  const int32_t* ptrIrow = a.I.ptr + (int64_t)I * (a.KD + 1);
-  const int64_t baseI = a.I.base[I];
   unsigned long long my_products = 0, my_head = 0, my_cells = 0;
   __syncthreads();
+  const int32_t J_first = J_first_s;
 
-  const int32_t J_first = a.self ? I : 0;
   for (int32_t J = J_first; J < a.J.T; ++J) {
-    const int64_t j0 = (int64_t)J * TILE;
-    const int nJ = (int)((a.J.n - j0) < TILE ? (a.J.n - j0) : TILE);
+    const int64_t j0 = a.J.tstart[J];
+    const int nJ = (int)(a.J.tstart[J + 1] - j0);
     const int64_t j1 = j0 + nJ;
-    const bool diag = a.self && (J == I);
-    if (tid < TILE) {
+
+    // ---- stage tile J ------------------------------------------------------
+    stage_block(a.J, J, KW, bJ, wpJ, cJ, rJ, vJ, tid);
+    if (tid < TILE_ROWS) {
       bool ok = tid < nJ;
       int64_t r = j0 + tid;
       normJ[tid] = ok ? a.J.norm[r] : 0.f;
@@ -121,109 +147,78 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       lenJ[tid] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
     }
 
-    // ---- 1. cursor tier --------------------------------------------------
+    // ---- 1. cursor tier (score tile only, no staging needed) ---------------
     for (int k = tid; k < nS; k += SCAN_THREADS) {
-      int32_t nx, cur;
-      if (k < S_CAP) { nx = s_nxt[k]; }
-      else { nx = a.snext_spill[2 * (sb + k) + 1]; }
+      int32_t nx = k < CAP_SI ? s_nxt[k] : a.snext_spill[sb + k];
       if (nx >= j1) continue;
-      cur = k < S_CAP ? s_cur[k] : a.snext_spill[2 * (sb + k)];
-      int4 r = a.I.srec[sb + k];
-      float x = __int_as_float(r.y);
-      const int rowbase = r.x * TSTRIDE;
+      const int4 r = a.I.srec[sb + k];
+      int32_t cur = a.scur[sb + k];
+      const float x = __int_as_float(r.y);
+      const int rowbase = r.x * TSTRIDE - (int)j0;
       do {
-        atomicAdd(&tile[rowbase + (int)(nx - j0)], x * a.S.val[cur]);
+        const int2 e = a.S.ent[cur];
+        atomicAdd(&tile[rowbase + nx], x * __int_as_float(e.y));
         ++my_products;
         ++cur;
-        nx = cur < r.w ? a.S.row[cur] : INT32_MAX;
+        nx = cur < r.w ? a.S.ent[cur].x : INT32_MAX;
       } while (nx < j1);
-      if (k < S_CAP) { s_cur[k] = cur; s_nxt[k] = nx; }
-      else { a.snext_spill[2 * (sb + k)] = cur; a.snext_spill[2 * (sb + k) + 1] = nx; }
+      a.scur[sb + k] = cur;
+      if (k < CAP_SI) s_nxt[k] = nx; else a.snext_spill[sb + k] = nx;
     }
+    __syncthreads();
 
-    // ---- 2. bitmap tier ----------------------------------------------------
-    const uint64_t* bmpJ = bmpJall + (int64_t)J * a.KW;
-    const int32_t* ptrJrow = a.J.ptr + (int64_t)J * (a.KD + 1);
-    const int64_t baseJ = a.J.base[J];
-    for (int32_t g0 = 0; g0 < a.KD; g0 += GROUP_FEATURES) {
-      const int g = g0 / GROUP_FEATURES;
-      __syncthreads();  // previous pass done with big[] / small[]
-      {
-        const int32_t w = g0 / 64 + wave;  // one bitmap word per wave
-        if (w < a.KW) {
-          uint64_t m = bmpI[w] & bmpJ[w];
-          if (m) {
-            bool has = (m >> lane) & 1ull;
-            Rec rec = {0, 0, 0};
-            bool is_big = false;
-            if (has) {
-              int32_t f = w * 64 + lane;
-              int32_t pa = ptrIrow[f], ca = ptrIrow[f + 1] - pa;
-              int32_t pb = ptrJrow[f], cb = ptrJrow[f + 1] - pb;
-              rec.a = (int32_t)(baseI + pa);
-              rec.b = (int32_t)(baseJ + pb);
-              rec.ab = ca | (cb << 16);
-              is_big = ca * cb >= BIG_PRODUCTS;
-            }
-            unsigned long long mb = __ballot(has && is_big);
-            unsigned long long ms = __ballot(has && !is_big);
-            unsigned int bb = 0, bs = 0;
-            if (lane == 0) {
-              if (mb) bb = atomicAdd(&n_big[g], (unsigned)__popcll(mb));
-              if (ms) bs = atomicAdd(&n_small[g], (unsigned)__popcll(ms));
-            }
-            bb = __shfl(bb, 0);
-            bs = __shfl(bs, 0);
-            if (has && is_big) big[bb + lane_rank(mb)] = rec;
-            if (has && !is_big) small[bs + lane_rank(ms)] = rec;
-          }
-        }
+    // ---- 2. bitmap tier: one bitmap word per wave and pass -------------------
+    for (int w = wave; w < KW; w += NW) {
+      const uint64_t wI = bI[w], wJ = bJ[w];
+      const uint64_t m = wI & wJ;
+      if (!m) continue;
+      const bool has = (m >> lane) & 1ull;
+      int aI = 0, ca = 0, aJ = 0, cb = 0;
+      if (has) {
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int rkI = wpI[w] + __popcll(wI & below);
+        const int rkJ = wpJ[w] + __popcll(wJ & below);
+        aI = cI[rkI];
+        ca = cI[rkI + 1] - aI;
+        aJ = cJ[rkJ];
+        cb = cJ[rkJ + 1] - aJ;
       }
-      __syncthreads();
-      const int nb = (int)n_big[g], ns = (int)n_small[g];
-      // large segments: one wave per feature, lanes over the flattened products
-      for (int r = wave; r < nb; r += NW) {
-        Rec rec = big[r];
-        const int ca = rec.ab & 0xffff, cb = rec.ab >> 16;
-        const int P = ca * cb;
-        const int dx = 64 / cb, dy = 64 - dx * cb;
-        int x = lane / cb, y = lane - x * cb;
-        for (int q = lane; q < P; q += 64) {
-          int il = a.I.drow[rec.a + x];
-          float u = a.I.dval[rec.a + x];
-          int jl = a.J.drow[rec.b + y];
-          float v = a.J.dval[rec.b + y];
-          atomicAdd(&tile[il * TSTRIDE + jl], u * v);
-          x += dx;
-          y += dy;
-          if (y >= cb) { y -= cb; ++x; }
-        }
-        if (lane == 0) my_products += (unsigned long long)P;
-      }
-      // small segments: one lane per feature
-      for (int r = tid; r < ns; r += SCAN_THREADS) {
-        Rec rec = small[r];
-        const int ca = rec.ab & 0xffff, cb = rec.ab >> 16;
+      const bool big = has && ca * cb >= BIG_PRODUCTS;
+      if (has && !big) {  // small segments: one lane per feature
         for (int x = 0; x < ca; ++x) {
-          int il = a.I.drow[rec.a + x];
-          float u = a.I.dval[rec.a + x];
-          float* trow = tile + il * TSTRIDE;
-          for (int y = 0; y < cb; ++y) atomicAdd(trow + a.J.drow[rec.b + y], u * a.J.dval[rec.b + y]);
+          const int rowbase = rI[aI + x] * TSTRIDE;
+          const float u = vI[aI + x];
+          for (int y = 0; y < cb; ++y) atomicAdd(&tile[rowbase + rJ[aJ + y]], u * vJ[aJ + y]);
         }
         my_products += (unsigned long long)(ca * cb);
       }
+      uint64_t mb = __ballot(big);
+      while (mb) {  // large segments: the wave walks the flattened products
+        const int b = __ffsll((long long)mb) - 1;
+        mb &= mb - 1ull;
+        const int fa = __shfl(aI, b), fca = __shfl(ca, b);
+        const int fb = __shfl(aJ, b), fcb = __shfl(cb, b);
+        const int P = fca * fcb;
+        const int dx = 64 / fcb, dy = 64 - dx * fcb;
+        int x = lane / fcb, y = lane - (lane / fcb) * fcb;
+        for (int q = lane; q < P; q += 64) {
+          atomicAdd(&tile[rI[fa + x] * TSTRIDE + rJ[fb + y]], vI[fa + x] * vJ[fb + y]);
+          x += dx;
+          y += dy;
+          if (y >= fcb) { y -= fcb; ++x; }
+        }
+        if (lane == 0) my_products += (unsigned long long)P;
+      }
     }
     __syncthreads();
-    // every pass of this tile pair has read its counters: zero them for the next
-    if (tid < N_GROUPS) { n_big[tid] = 0; n_small[tid] = 0; }
 
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
-      for (int c = tid; c < TILE * TILE; c += SCAN_THREADS) {
-        const int il = c / TILE, jl = c - (c / TILE) * TILE;
-        if (il >= nI || jl >= nJ || (diag && jl <= il)) continue;
+      for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
+        const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
+        if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
         float* cell = &tile[il * TSTRIDE + jl];
-        float part = *cell;
+        const float part = *cell;
         if (part + hnI[il] * hnJ[jl] < a.t_lo) continue;  // pruned: head cannot reach t
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
@@ -235,12 +230,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
 
     // ---- 4. threshold, compaction, reset -------------------------------------
-    for (int c = tid; c < TILE * TILE; c += SCAN_THREADS) {
-      const int il = c / TILE, jl = c - (c / TILE) * TILE;
+    for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
+      const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
       float* cell = &tile[il * TSTRIDE + jl];
       const float s = *cell;
       *cell = 0.f;
-      const bool valid = il < nI && jl < nJ && !(diag && jl <= il);
+      const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
       my_cells += valid ? 1 : 0;
       bool emit = false, to_band = false;
       if (valid && s >= a.t_lo) {
