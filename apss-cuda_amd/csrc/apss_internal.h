@@ -29,11 +29,8 @@ constexpr int KD_MAX = 4096;                 // features in the bitmap tier
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
 constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
-constexpr int CAP_SI = 6144;                 // cursor-tier entries of a band kept in LDS
-constexpr int CAP_REC = 2048;                // common-feature records per pass
+constexpr int CAP_SI = 8192;                 // cursor-tier entries of a band kept in LDS
 constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
-constexpr int BIG_PRODUCTS = 32;             // common features with >= this many
-                                             // products get a whole wave
 
 // Feature classes in fmap[] (values >= 0 are bitmap-tier indices).
 constexpr int32_t F_HEAD = -1;    // id < p: head (forward list / MFMA)
@@ -72,6 +69,7 @@ struct TileSide {
   int64_t n = 0;                  // rows
   int32_t T = 0;                  // tiles (or bands)
   const int64_t* tstart = nullptr;  // T+1 first row of each tile
+  const int32_t* row2tile = nullptr;  // n, tile of each row
   const int64_t* rp = nullptr;    // CSR (device)
   const int32_t* col = nullptr;
   const float* val = nullptr;
@@ -121,7 +119,7 @@ struct ScanArgs {
   int32_t pad2_;
   unsigned long long* stats; // [0] products, [1] head pairs, [2] cells
   int32_t* scur;             // cursor position of every cursor record (I side)
-  int32_t* snext_spill;      // next row of records beyond CAP_SI
+  int32_t* snext_spill;      // next tile of records beyond CAP_SI
 };
 
 // Arguments of the per-tile bitmap-tier build kernels.

@@ -25,6 +25,7 @@ __global__ void k_scan_i64(int64_t* a, int64_t n);
 __global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
 __global__ void k_tile_compact(BuildSide s);
+__global__ void k_row2tile(const int64_t* tstart, int32_t* row2tile);
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
@@ -89,6 +90,7 @@ struct Side {
 struct TileSet {
   int32_t T = 0;
   int64_t* tstart = nullptr;
+  int32_t* row2tile = nullptr;
   uint64_t* bmp = nullptr;
   uint16_t* wpre = nullptr;
   int64_t* cbase = nullptr;

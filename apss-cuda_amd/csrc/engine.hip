@@ -100,6 +100,7 @@ TileSide make_view(const Side& s, const TileSet& ts) {
   v.n = s.n;
   v.T = ts.T;
   v.tstart = ts.tstart;
+  v.row2tile = ts.row2tile;
   v.rp = s.rp;
   v.col = s.col;
   v.val = s.val;
@@ -338,6 +339,12 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   ts.T = T;
   ts.tstart = ar.alloc<int64_t>(T + 1);
   APSS_HIP(hipMemcpyAsync(ts.tstart, cuts.data(), sizeof(int64_t) * (T + 1), hipMemcpyHostToDevice, st));
+  if (T >= 0xffff) throw Error{APSS_E_UNSUPPORTED, "more than 65534 tiles"};
+  ts.row2tile = ar.alloc<int32_t>(s.n);
+  if (T > 0) {
+    hipLaunchKernelGGL(k_row2tile, dim3(T), dim3(256), 0, st, ts.tstart, ts.row2tile);
+    APSS_HIP(hipGetLastError());
+  }
   ts.bmp = ar.alloc<uint64_t>((int64_t)T * tr.KW);
   ts.wpre = ar.alloc<uint16_t>((int64_t)T * tr.KW);
   ts.cbase = ar.alloc<int64_t>((int64_t)T + 1);
@@ -543,7 +550,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->ms_upload = tm.stop_ms();
   tm.start();
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_SI, true);
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true);
   TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false);
   out->ms_index = tm.stop_ms();
   RunConfig rc;

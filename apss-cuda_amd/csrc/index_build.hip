@@ -384,4 +384,11 @@ __global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {
   if (threadIdx.x == 0) crow[wp[s.KW]] = prow[s.KD];
 }
 
+// Tile index of every row.
+__global__ __launch_bounds__(256) void k_row2tile(const int64_t* __restrict__ tstart,
+                                                  int32_t* __restrict__ row2tile) {
+  const int t = blockIdx.x;
+  for (int64_t r = tstart[t] + threadIdx.x; r < tstart[t + 1]; r += blockDim.x) row2tile[r] = t;
+}
+
 }  // namespace apss

@@ -48,39 +48,65 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
   return s;
 }
 
-// Copy one tile block (bitmap words, word prefixes, compact starts, entries)
-// from HBM into LDS.  Block sizes are padded to multiples of 4 entries so the
-// entries move as 32-bit / 128-bit words.
-__device__ __forceinline__ void stage_block(const TileSide& S, int32_t t, int KW, uint64_t* bmp,
-                                            uint16_t* wpre, uint16_t* cst, uint8_t* rows,
-                                            float* vals, int tid) {
-  for (int w = tid; w < KW; w += SCAN_THREADS) {
-    bmp[w] = S.bmp[(int64_t)t * KW + w];
-    wpre[w] = S.wpre[(int64_t)t * KW + w];
-  }
+// Tile-block staging.  Block sizes are padded to multiples of 4 entries so
+// the entries move as 32-bit / 128-bit words.  The loads for the next tile
+// are issued into registers before the current tile's product phase and
+// written to LDS after it, so their latency hides behind the products.
+constexpr int CST_PER_THREAD = (KD_MAX + 1 + SCAN_THREADS - 1) / SCAN_THREADS;
+static_assert(CAP_J / 4 <= SCAN_THREADS, "one entry quad per thread");
+
+struct Prefetch {
+  uint64_t bmp;
+  uint16_t wpre;
+  int32_t cst[CST_PER_THREAD];
+  uint32_t rows;
+  float4 vals;
+  float norm, hnorm;
+  int32_t len;
+  int32_t ncs, nd4, nrows;
+};
+
+__device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int KW, int tid,
+                                              Prefetch& pf) {
   const int64_t c0 = S.cbase[t];
-  const int ncs = (int)(S.cbase[t + 1] - c0);  // present features + 1
-  for (int k = tid; k < ncs; k += SCAN_THREADS) cst[k] = (uint16_t)S.cst[c0 + k];
+  pf.ncs = (int)(S.cbase[t + 1] - c0);  // present features + 1
   const int64_t d0 = S.dbase[t];
-  const int nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
-  const uint32_t* r4 = reinterpret_cast<const uint32_t*>(S.drow + d0);
-  const float4* v4 = reinterpret_cast<const float4*>(S.dval + d0);
-  for (int k = tid; k < nd4; k += SCAN_THREADS) {
-    reinterpret_cast<uint32_t*>(rows)[k] = r4[k];
-    reinterpret_cast<float4*>(vals)[k] = v4[k];
+  pf.nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
+  const int64_t r0 = S.tstart[t];
+  pf.nrows = (int)(S.tstart[t + 1] - r0);
+  if (tid < KW) {
+    pf.bmp = S.bmp[(int64_t)t * KW + tid];
+    pf.wpre = S.wpre[(int64_t)t * KW + tid];
+  }
+#pragma unroll
+  for (int i = 0; i < CST_PER_THREAD; ++i) {
+    const int k = tid + i * SCAN_THREADS;
+    if (k < pf.ncs) pf.cst[i] = S.cst[c0 + k];
+  }
+  if (tid < pf.nd4) {
+    pf.rows = reinterpret_cast<const uint32_t*>(S.drow + d0)[tid];
+    pf.vals = reinterpret_cast<const float4*>(S.dval + d0)[tid];
+  }
+  if (tid < pf.nrows) {
+    const int64_t r = r0 + tid;
+    pf.norm = S.norm[r];
+    pf.hnorm = S.hnorm[r];
+    pf.len = (int32_t)(S.rp[r + 1] - S.rp[r]);
   }
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
+  constexpr int NW = SCAN_THREADS / 64;
   __shared__ float tile[BAND_ROWS * TSTRIDE];
   __shared__ uint64_t bI[KW_MAX], bJ[KW_MAX];
   __shared__ uint16_t wpI[KW_MAX], wpJ[KW_MAX];
   __shared__ uint16_t cI[KD_MAX + 1], cJ[KD_MAX + 1];
   __shared__ __align__(16) uint8_t rI[CAP_I];
-  __shared__ __align__(16) uint8_t rJ[CAP_J];
   __shared__ __align__(16) float vI[CAP_I];
-  __shared__ __align__(16) float vJ[CAP_J];
-  __shared__ int32_t s_nxt[CAP_SI];
+  __shared__ __align__(16) int2 eJ[CAP_J];          // {local row, value bits}
+  __shared__ uint16_t s_nt[CAP_SI];                  // next J tile of each cursor
+  __shared__ uint2 wrec[NW][64];                     // per-wave common-feature records
+  __shared__ int32_t wincl[NW][64];                  // per-wave inclusive product prefix
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
   __shared__ int32_t J_first_s;
@@ -89,7 +115,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid / 64;
-  constexpr int NW = SCAN_THREADS / 64;
   const int KW = a.KW;
 
   const int32_t I = a.band_first + blockIdx.x;
@@ -98,7 +123,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int nI = (int)(a.I.tstart[I + 1] - i0);
 
   // ---- band set-up -------------------------------------------------------
-  stage_block(a.I, I, KW, bI, wpI, cI, rI, vI, tid);
+  {
+    for (int w = tid; w < KW; w += SCAN_THREADS) {
+      bI[w] = a.I.bmp[(int64_t)I * KW + w];
+      wpI[w] = a.I.wpre[(int64_t)I * KW + w];
+    }
+    const int64_t c0 = a.I.cbase[I];
+    const int ncs = (int)(a.I.cbase[I + 1] - c0);
+    for (int k = tid; k < ncs; k += SCAN_THREADS) cI[k] = (uint16_t)a.I.cst[c0 + k];
+    const int64_t d0 = a.I.dbase[I];
+    const int nd4 = (int)((a.I.dbase[I + 1] - d0) >> 2);
+    for (int k = tid; k < nd4; k += SCAN_THREADS) {
+      reinterpret_cast<uint32_t*>(rI)[k] = reinterpret_cast<const uint32_t*>(a.I.drow + d0)[k];
+      reinterpret_cast<float4*>(vI)[k] = reinterpret_cast<const float4*>(a.I.dval + d0)[k];
+    }
+  }
   for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0.f;
   if (tid < BAND_ROWS) {
     bool ok = tid < nI;
@@ -124,57 +163,74 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int nS = (int)(a.I.sbase[I + 1] - sb);
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     int4 r = a.I.srec[sb + k];
-    int32_t nx = r.z < r.w ? a.S.ent[r.z].x : INT32_MAX;
+    int32_t nt = r.z < r.w ? a.J.row2tile[a.S.ent[r.z].x] : 0xffff;
     a.scur[sb + k] = r.z;
-    if (k < CAP_SI) s_nxt[k] = nx; else a.snext_spill[sb + k] = nx;
+    if (k < CAP_SI) s_nt[k] = (uint16_t)nt; else a.snext_spill[sb + k] = nt;
   }
   unsigned long long my_products = 0, my_head = 0, my_cells = 0;
   __syncthreads();
   const int32_t J_first = J_first_s;
 
+  Prefetch pf;
+  if (J_first < a.J.T) prefetch_tile(a.J, J_first, KW, tid, pf);
+
   for (int32_t J = J_first; J < a.J.T; ++J) {
     const int64_t j0 = a.J.tstart[J];
-    const int nJ = (int)(a.J.tstart[J + 1] - j0);
+    const int nJ = pf.nrows;
     const int64_t j1 = j0 + nJ;
 
-    // ---- stage tile J ------------------------------------------------------
-    stage_block(a.J, J, KW, bJ, wpJ, cJ, rJ, vJ, tid);
+    // ---- write the prefetched tile J into LDS --------------------------------
+    if (tid < KW) { bJ[tid] = pf.bmp; wpJ[tid] = pf.wpre; }
+#pragma unroll
+    for (int i = 0; i < CST_PER_THREAD; ++i) {
+      const int k = tid + i * SCAN_THREADS;
+      if (k < pf.ncs) cJ[k] = (uint16_t)pf.cst[i];
+    }
+    if (tid < pf.nd4) {
+      const float vv[4] = {pf.vals.x, pf.vals.y, pf.vals.z, pf.vals.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        eJ[4 * tid + q] = make_int2((int)((pf.rows >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
+    }
     if (tid < TILE_ROWS) {
-      bool ok = tid < nJ;
-      int64_t r = j0 + tid;
-      normJ[tid] = ok ? a.J.norm[r] : 0.f;
-      hnJ[tid] = ok ? a.J.hnorm[r] : 0.f;
-      lenJ[tid] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
+      const bool ok = tid < nJ;
+      normJ[tid] = ok ? pf.norm : 0.f;
+      hnJ[tid] = ok ? pf.hnorm : 0.f;
+      lenJ[tid] = ok ? pf.len : 0;
     }
 
-    // ---- 1. cursor tier (score tile only, no staging needed) ---------------
+    // ---- 1. cursor tier (score tile only) -----------------------------------
     for (int k = tid; k < nS; k += SCAN_THREADS) {
-      int32_t nx = k < CAP_SI ? s_nxt[k] : a.snext_spill[sb + k];
-      if (nx >= j1) continue;
+      const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
+      if (nt != J) continue;
       const int4 r = a.I.srec[sb + k];
       int32_t cur = a.scur[sb + k];
       const float x = __int_as_float(r.y);
       const int rowbase = r.x * TSTRIDE - (int)j0;
-      do {
-        const int2 e = a.S.ent[cur];
-        atomicAdd(&tile[rowbase + nx], x * __int_as_float(e.y));
+      int2 e = a.S.ent[cur];
+      int32_t ntn = 0xffff;
+      for (;;) {
+        atomicAdd(&tile[rowbase + e.x], x * __int_as_float(e.y));
         ++my_products;
-        ++cur;
-        nx = cur < r.w ? a.S.ent[cur].x : INT32_MAX;
-      } while (nx < j1);
+        if (++cur >= r.w) break;
+        e = a.S.ent[cur];
+        if (e.x >= j1) { ntn = a.J.row2tile[e.x]; break; }
+      }
       a.scur[sb + k] = cur;
-      if (k < CAP_SI) s_nxt[k] = nx; else a.snext_spill[sb + k] = nx;
+      if (k < CAP_SI) s_nt[k] = (uint16_t)ntn; else a.snext_spill[sb + k] = ntn;
     }
     __syncthreads();
 
-    // ---- 2. bitmap tier: one bitmap word per wave and pass -------------------
+    // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
+    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tid, pf);
+
+    // ---- 2. bitmap tier: wave-local merge path over the products -----------
     for (int w = wave; w < KW; w += NW) {
       const uint64_t wI = bI[w], wJ = bJ[w];
       const uint64_t m = wI & wJ;
       if (!m) continue;
-      const bool has = (m >> lane) & 1ull;
       int aI = 0, ca = 0, aJ = 0, cb = 0;
-      if (has) {
+      if ((m >> lane) & 1ull) {
         const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
         const int rkI = wpI[w] + __popcll(wI & below);
         const int rkJ = wpJ[w] + __popcll(wJ & below);
@@ -183,35 +239,56 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         aJ = cJ[rkJ];
         cb = cJ[rkJ + 1] - aJ;
       }
-      const bool big = has && ca * cb >= BIG_PRODUCTS;
-      if (has && !big) {  // small segments: one lane per feature
-        for (int x = 0; x < ca; ++x) {
-          const int rowbase = rI[aI + x] * TSTRIDE;
-          const float u = vI[aI + x];
-          for (int y = 0; y < cb; ++y) atomicAdd(&tile[rowbase + rJ[aJ + y]], u * vJ[aJ + y]);
-        }
-        my_products += (unsigned long long)(ca * cb);
+      int incl = ca * cb;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
       }
-      uint64_t mb = __ballot(big);
-      while (mb) {  // large segments: the wave walks the flattened products
-        const int b = __ffsll((long long)mb) - 1;
-        mb &= mb - 1ull;
-        const int fa = __shfl(aI, b), fca = __shfl(ca, b);
-        const int fb = __shfl(aJ, b), fcb = __shfl(cb, b);
-        const int P = fca * fcb;
-        const int dx = 64 / fcb, dy = 64 - dx * fcb;
-        int x = lane / fcb, y = lane - (lane / fcb) * fcb;
-        for (int q = lane; q < P; q += 64) {
-          atomicAdd(&tile[rI[fa + x] * TSTRIDE + rJ[fb + y]], vI[fa + x] * vJ[fb + y]);
-          x += dx;
-          y += dy;
-          if (y >= fcb) { y -= fcb; ++x; }
+      wrec[wave][lane] = make_uint2((uint32_t)aI | ((uint32_t)aJ << 16), (uint32_t)ca | ((uint32_t)cb << 8));
+      wincl[wave][lane] = incl;
+      __builtin_amdgcn_wave_barrier();
+      const int total = __shfl(incl, 63);
+      const int chunk = (total + 63) >> 6;
+      int q = lane * chunk;
+      const int qend = min(q + chunk, total);
+      if (q < qend) {
+        int lo = 0, hi = 63;  // first feature whose inclusive prefix exceeds q
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (wincl[wave][mid] > q) hi = mid; else lo = mid + 1;
         }
-        if (lane == 0) my_products += (unsigned long long)P;
+        int b = lo;
+        uint2 r = wrec[wave][b];
+        int fa = r.x & 0xffff, fb = r.x >> 16, fca = r.y & 0xff, fcb = r.y >> 8;
+        int loc = q - (wincl[wave][b] - fca * fcb);
+        int x = (int)((float)loc * __frcp_rn((float)fcb));
+        int y = loc - x * fcb;
+        if (y < 0) { --x; y += fcb; }
+        if (y >= fcb) { ++x; y -= fcb; }
+        int ib = rI[fa + x] * TSTRIDE;
+        float u = vI[fa + x];
+        for (;;) {
+          const int2 e = eJ[fb + y];
+          atomicAdd(&tile[ib + e.x], u * __int_as_float(e.y));
+          if (++q == qend) break;
+          if (++y == fcb) {
+            y = 0;
+            if (++x == fca) {  // next common feature of this word
+              x = 0;
+              b = __ffsll((long long)(m & (~0ull << b) & ~(1ull << b))) - 1;
+              r = wrec[wave][b];
+              fa = r.x & 0xffff; fb = r.x >> 16; fca = r.y & 0xff; fcb = r.y >> 8;
+            }
+            ib = rI[fa + x] * TSTRIDE;
+            u = vI[fa + x];
+          }
+        }
+        my_products += (unsigned long long)(qend - lane * chunk);
       }
+      __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
       for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
