@@ -417,10 +417,15 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
                             const RunConfig& rc) {
   ScanOutput so;
   const int64_t maxlen = std::max(I.maxlen, J.maxlen);
-  const double maxn = (double)std::max(I.maxnorm, 1e-30f) * (double)std::max(J.maxnorm, 1e-30f);
-  const double eps_max = gamma32(maxlen + 4) * maxn * (1.0 + 1e-6) + 1e-44 * (maxlen + 4);
-  float t_lo = (float)(rc.t - 2.0 * eps_max);
-  if ((double)t_lo > rc.t - 2.0 * eps_max) t_lo = nextafterf(t_lo, -INFINITY);
+  const double mI = std::max((double)I.maxnorm, 1e-30), mJ = std::max((double)J.maxnorm, 1e-30);
+  // fixed-point scale 2^S with 2^S * max|x||y| <= 2^30 (partial sums < 2^31)
+  int S = 30 - (int)std::ceil(std::log2(mI * mJ));
+  S = std::max(-100, std::min(S, 100));
+  const double qscale = std::ldexp(1.0, S), qinv = std::ldexp(1.0, -S);
+  const double head_eps = tr.p > 0 ? gamma32(maxlen + 1) * mI * mJ : 0.0;
+  const double eps_max = cell_error_bound((int)maxlen, mI, mJ, qinv, head_eps);
+  const double qlo = std::floor((rc.t - 2.0 * eps_max) * qscale);
+  const int32_t q_lo = (int32_t)std::max(-2147483647.0, std::min(2147483647.0, qlo));
 
   // pair-space size bounds the output capacity
   const double cells = rc.self ? 0.5 * (double)I.n * (double)(I.n - 1) : (double)I.n * (double)J.n;
@@ -448,7 +453,10 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.self = rc.self;
     a.head_mode = tr.p > 0 ? 1 : 0;
     a.t = rc.t;
-    a.t_lo = t_lo;
+    a.qscale = (float)qscale;
+    a.qinv = qinv;
+    a.q_lo = q_lo;
+    a.score_rel = 1e-6;
     a.i_off = rc.i_off;
     a.j_off = rc.j_off;
     a.out = out;

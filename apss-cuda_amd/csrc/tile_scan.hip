@@ -97,7 +97,7 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int 
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   constexpr int NW = SCAN_THREADS / 64;
-  __shared__ float tile[BAND_ROWS * TSTRIDE];
+  __shared__ int32_t tile[BAND_ROWS * TSTRIDE];   // fixed-point scores (x 2^S)
   __shared__ uint64_t bI[KW_MAX], bJ[KW_MAX];
   __shared__ uint16_t wpI[KW_MAX], wpJ[KW_MAX];
   __shared__ uint16_t cI[KD_MAX + 1], cJ[KD_MAX + 1];
@@ -135,10 +135,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const int nd4 = (int)((a.I.dbase[I + 1] - d0) >> 2);
     for (int k = tid; k < nd4; k += SCAN_THREADS) {
       reinterpret_cast<uint32_t*>(rI)[k] = reinterpret_cast<const uint32_t*>(a.I.drow + d0)[k];
-      reinterpret_cast<float4*>(vI)[k] = reinterpret_cast<const float4*>(a.I.dval + d0)[k];
+      float4 v = reinterpret_cast<const float4*>(a.I.dval + d0)[k];
+      v.x *= a.qscale; v.y *= a.qscale; v.z *= a.qscale; v.w *= a.qscale;  // exact: power of 2
+      reinterpret_cast<float4*>(vI)[k] = v;
     }
   }
-  for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0.f;
+  for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid < BAND_ROWS) {
     bool ok = tid < nI;
     int64_t r = i0 + tid;
@@ -205,12 +207,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if (nt != J) continue;
       const int4 r = a.I.srec[sb + k];
       int32_t cur = a.scur[sb + k];
-      const float x = __int_as_float(r.y);
+      const float x = __int_as_float(r.y) * a.qscale;
       const int rowbase = r.x * TSTRIDE - (int)j0;
       int2 e = a.S.ent[cur];
       int32_t ntn = 0xffff;
       for (;;) {
-        atomicAdd(&tile[rowbase + e.x], x * __int_as_float(e.y));
+        atomicAdd(&tile[rowbase + e.x], __float2int_rn(x * __int_as_float(e.y)));
         ++my_products;
         if (++cur >= r.w) break;
         e = a.S.ent[cur];
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         float u = vI[fa + x];
         for (;;) {
           const int2 e = eJ[fb + y];
-          atomicAdd(&tile[ib + e.x], u * __int_as_float(e.y));
+          atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
           if (++q == qend) break;
           if (++y == fcb) {
             y = 0;
@@ -294,13 +296,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
         const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
         if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
-        float* cell = &tile[il * TSTRIDE + jl];
-        const float part = *cell;
-        if (part + hnI[il] * hnJ[jl] < a.t_lo) continue;  // pruned: head cannot reach t
+        int32_t* cell = &tile[il * TSTRIDE + jl];
+        const int32_t part = *cell;
+        // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
+        const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jl];
+        if (bound + cell_error_bound(lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl], normI[il], normJ[jl],
+                                     a.qinv, 0.0) < a.t)
+          continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
-        *cell = part + head_dot(a.I.col + ai, a.I.val + ai, a.I.rp[ri + 1] - ai, a.J.col + aj,
-                                a.J.val + aj, a.J.rp[rj + 1] - aj, a.p);
+        const float h = head_dot(a.I.col + ai, a.I.val + ai, a.I.rp[ri + 1] - ai, a.J.col + aj,
+                                 a.J.val + aj, a.J.rp[rj + 1] - aj, a.p);
+        *cell = part + __float2int_rn(h * a.qscale);
         ++my_head;
       }
       __syncthreads();
@@ -309,19 +316,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // ---- 4. threshold, compaction, reset -------------------------------------
     for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
       const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
-      float* cell = &tile[il * TSTRIDE + jl];
-      const float s = *cell;
-      *cell = 0.f;
+      int32_t* cell = &tile[il * TSTRIDE + jl];
+      const int32_t q = *cell;
+      *cell = 0;
       const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
       my_cells += valid ? 1 : 0;
       bool emit = false, to_band = false;
-      if (valid && s >= a.t_lo) {
-        const int k = (lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl]) + 2;
-        const double eps = gamma32(k) * (double)normI[il] * (double)normJ[jl] * (1.0 + 1e-6) +
-                           (double)k * 1.401298464324817e-45;
-        const double sd = (double)s;
+      double sd = 0.0;
+      if (valid && q >= a.q_lo) {
+        const int k = lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl];
+        const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jl] : 0.0;
+        const double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
+        sd = (double)q * a.qinv;
         if (sd - eps >= a.t) {
-          if (a.rescore_all) to_band = true; else emit = true;
+          // certain hit; its score is final only if the bound meets the relative tolerance
+          if (a.rescore_all || eps > a.score_rel * (sd - eps)) to_band = true; else emit = true;
         } else if (sd + eps >= a.t) {
           to_band = true;
         }
@@ -332,7 +341,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         apss_pair pr;
         pr.i = (uint32_t)(i0 + il + a.i_off);
         pr.j = (uint32_t)(j0 + jl + a.j_off);
-        pr.score = s;
+        pr.score = (float)sd;
         a.out[slot] = pr;
       }
       long long bslot = wave_reserve(a.band_cnt, to_band, &tot);
