@@ -52,6 +52,7 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 // the entries move as 32-bit / 128-bit words.  The loads for the next tile
 // are issued into registers before the current tile's product phase and
 // written to LDS after it, so their latency hides behind the products.
+constexpr int OBUF_CAP = 512;               // LDS output buffer (pairs)
 constexpr int CST_PER_THREAD = (KD_MAX + 1 + SCAN_THREADS - 1) / SCAN_THREADS;
 static_assert(CAP_J / 4 <= SCAN_THREADS, "one entry quad per thread");
 
@@ -109,6 +110,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ int32_t wincl[NW][64];                  // per-wave inclusive product prefix
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
+  __shared__ apss_pair obuf[OBUF_CAP];                // emitted pairs waiting for a flush
+  __shared__ unsigned int obuf_n;
+  __shared__ unsigned long long obuf_base;
   __shared__ int32_t J_first_s;
   __shared__ unsigned long long acc_products, acc_head, acc_cells;
 
@@ -149,7 +153,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
   }
   if (tid == 0) {
-    acc_products = 0; acc_head = 0; acc_cells = 0;
+    acc_products = 0; acc_head = 0; acc_cells = 0; obuf_n = 0;
     int32_t jf = 0;
     if (a.self) {  // first tile whose rows end after the band's first row
       int32_t lo = 0, hi = a.J.T;
@@ -335,20 +339,49 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           to_band = true;
         }
       }
-      int tot;
-      long long slot = wave_reserve(a.out_cnt, emit, &tot);
-      if (emit && slot < a.out_cap) {
+      // emitted pairs go to the workgroup's LDS buffer (one LDS atomic per
+      // wave); a wave that finds the buffer full reserves in HBM directly
+      const unsigned long long me = __ballot(emit);
+      if (me) {
+        const int tot = __popcll(me);
+        const int leader = __ffsll((long long)me) - 1;
+        unsigned int lb = 0;
+        if (lane == leader) lb = atomicAdd(&obuf_n, (unsigned)tot);
+        lb = __shfl(lb, leader);
         apss_pair pr;
         pr.i = (uint32_t)(i0 + il + a.i_off);
         pr.j = (uint32_t)(j0 + jl + a.j_off);
         pr.score = (float)sd;
-        a.out[slot] = pr;
+        // every buffer slot below OBUF_CAP is written by the wave that reserved
+        // it; the part of a reservation beyond the buffer goes straight to HBM
+        const unsigned int pos = lb + (unsigned)lane_rank(me);
+        if (emit && pos < OBUF_CAP) obuf[pos] = pr;
+        if (lb + tot > OBUF_CAP) {
+          int t2;
+          const bool spill = emit && pos >= OBUF_CAP;
+          long long slot = wave_reserve(a.out_cnt, spill, &t2);
+          if (spill && slot < a.out_cap) a.out[slot] = pr;
+        }
       }
+      int tot;
       long long bslot = wave_reserve(a.band_cnt, to_band, &tot);
       if (to_band && bslot < a.band_cap)
         a.band[bslot] = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
     }
     __syncthreads();
+    // flush the LDS output buffer when half full, and after the last tile
+    const unsigned int nbuf = obuf_n;
+    if (nbuf >= OBUF_CAP / 2 || (J + 1 == a.J.T && nbuf > 0)) {
+      const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
+      if (tid == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
+      __syncthreads();
+      const unsigned long long ob = obuf_base;
+      for (unsigned int k = tid; k < nf; k += SCAN_THREADS)
+        if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
+      __syncthreads();
+      if (tid == 0) obuf_n = 0;
+      __syncthreads();
+    }
   }
 
   atomicAdd(&acc_products, my_products);
