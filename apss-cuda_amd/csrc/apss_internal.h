@@ -84,6 +84,7 @@ struct TileSide {
   const int64_t* dbase = nullptr; // T+1, start of each tile block
   const uint8_t* drow = nullptr;  // local row of each bitmap-tier entry
   const float* dval = nullptr;
+  const uint16_t* dfeat = nullptr;  // bitmap index of each bitmap-tier entry
   // cursor-tier entries of this side (one record per entry, grouped by tile)
   const int64_t* sbase = nullptr; // T+1
   const int4* srec = nullptr;     // {local row, float bits of x, first cursor, end}
@@ -149,6 +150,7 @@ struct BuildSide {
   int64_t* stot;              // T+1
   uint8_t* drow;
   float* dval;
+  uint16_t* dfeat;
   int4* srec;
   const int64_t* sptr;        // corpus cursor lists
   const int2* sent;

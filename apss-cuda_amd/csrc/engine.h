@@ -98,6 +98,7 @@ struct TileSet {
   int64_t* dbase = nullptr;
   uint8_t* drow = nullptr;
   float* dval = nullptr;
+  uint16_t* dfeat = nullptr;
   int64_t* sbase = nullptr;
   int4* srec = nullptr;
   int64_t nD = 0, nS = 0;

@@ -113,6 +113,7 @@ TileSide make_view(const Side& s, const TileSet& ts) {
   v.dbase = ts.dbase;
   v.drow = ts.drow;
   v.dval = ts.dval;
+  v.dfeat = ts.dfeat;
   v.sbase = ts.sbase;
   v.srec = ts.srec;
   return v;
@@ -238,14 +239,14 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   tr.n_bitmap = (int32_t)cand.size();
   tr.KD = std::max(64, (tr.n_bitmap + 63) / 64 * 64);
   tr.KW = tr.KD / 64;
-  // Frequency rank k goes to word k % KW, bit k / KW: every bitmap word mixes
-  // frequent and rare features, so the waves that own words get equal work.
+  // The bitmap index of a feature is its frequency rank, so every tile block
+  // holds its segments from the most to the least frequent feature: threads
+  // that own neighbouring entries see segments of similar length.
   std::vector<int32_t> fm((size_t)d, F_IGNORE);
   for (int64_t c = 0; c < std::min<int64_t>(p, d); ++c) fm[c] = F_HEAD;
   for (int64_t c = p; c < d; ++c)
     if (h[c] >= need) fm[c] = F_CURSOR;
-  for (size_t k = 0; k < cand.size(); ++k)
-    fm[cand[k]] = (int32_t)((k % tr.KW) * 64 + k / tr.KW);
+  for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
   std::vector<int64_t> sp((size_t)d + 1, 0);
   std::vector<int32_t> small_lists, large_lists;
   for (int64_t c = 0; c < d; ++c) {
@@ -393,10 +394,12 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   ts.nS = records ? tot[1] : 0;
   ts.drow = ar.alloc<uint8_t>(ts.nD);
   ts.dval = ar.alloc<float>(ts.nD);
+  ts.dfeat = ar.alloc<uint16_t>(ts.nD);
   ts.cst = ar.alloc<int32_t>(tot[2]);
   ts.srec = ar.alloc<int4>(records ? tot[1] : 0);
   b.drow = ts.drow;
   b.dval = ts.dval;
+  b.dfeat = ts.dfeat;
   b.cst = ts.cst;
   b.srec = records ? ts.srec : nullptr;
   if (T > 0) {

@@ -348,6 +348,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
       int32_t q = atomicAdd(&prow[f + 1], 1);
       s.drow[dbase + q] = (uint8_t)lo;
       s.dval[dbase + q] = s.val[e];
+      s.dfeat[dbase + q] = (uint16_t)f;
     } else if (s.srec) {  // cursor records only for the band (I) side
       unsigned int q = atomicAdd(&sfill, 1u);
       int64_t a = s.sptr[c], b = s.sptr[c + 1];

@@ -53,17 +53,17 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 // are issued into registers before the current tile's product phase and
 // written to LDS after it, so their latency hides behind the products.
 constexpr int OBUF_CAP = 512;               // LDS output buffer (pairs)
-constexpr int CST_PER_THREAD = (KD_MAX + 1 + SCAN_THREADS - 1) / SCAN_THREADS;
+constexpr int I_PER_THREAD = CAP_I / SCAN_THREADS;
+constexpr int CST_PAIRS = (KD_MAX + 2 + 2 * SCAN_THREADS - 1) / (2 * SCAN_THREADS);
 static_assert(CAP_J / 4 <= SCAN_THREADS, "one entry quad per thread");
+static_assert(KD_MAX < 65536 && CAP_J < 65536, "16-bit segment starts");
 
 struct Prefetch {
   uint64_t bmp;
-  uint16_t wpre;
-  int32_t cst[CST_PER_THREAD];
+  uint32_t wpre;
+  uint32_t cst[CST_PAIRS];  // two 16-bit segment starts per register
   uint32_t rows;
   float4 vals;
-  float norm, hnorm;
-  int32_t len;
   int32_t ncs, nd4, nrows;
 };
 
@@ -73,52 +73,42 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int 
   pf.ncs = (int)(S.cbase[t + 1] - c0);  // present features + 1
   const int64_t d0 = S.dbase[t];
   pf.nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
-  const int64_t r0 = S.tstart[t];
-  pf.nrows = (int)(S.tstart[t + 1] - r0);
+  pf.nrows = (int)(S.tstart[t + 1] - S.tstart[t]);
   if (tid < KW) {
     pf.bmp = S.bmp[(int64_t)t * KW + tid];
     pf.wpre = S.wpre[(int64_t)t * KW + tid];
   }
 #pragma unroll
-  for (int i = 0; i < CST_PER_THREAD; ++i) {
-    const int k = tid + i * SCAN_THREADS;
-    if (k < pf.ncs) pf.cst[i] = S.cst[c0 + k];
+  for (int i = 0; i < CST_PAIRS; ++i) {
+    const int k = 2 * (tid + i * SCAN_THREADS);
+    uint32_t lo = k < pf.ncs ? (uint32_t)S.cst[c0 + k] : 0u;
+    uint32_t hi = k + 1 < pf.ncs ? (uint32_t)S.cst[c0 + k + 1] : 0u;
+    pf.cst[i] = lo | (hi << 16);
   }
   if (tid < pf.nd4) {
     pf.rows = reinterpret_cast<const uint32_t*>(S.drow + d0)[tid];
     pf.vals = reinterpret_cast<const float4*>(S.dval + d0)[tid];
   }
-  if (tid < pf.nrows) {
-    const int64_t r = r0 + tid;
-    pf.norm = S.norm[r];
-    pf.hnorm = S.hnorm[r];
-    pf.len = (int32_t)(S.rp[r + 1] - S.rp[r]);
-  }
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
-  constexpr int NW = SCAN_THREADS / 64;
-  __shared__ int32_t tile[BAND_ROWS * TSTRIDE];   // fixed-point scores (x 2^S)
-  __shared__ uint64_t bI[KW_MAX], bJ[KW_MAX];
-  __shared__ uint16_t wpI[KW_MAX], wpJ[KW_MAX];
-  __shared__ uint16_t cI[KD_MAX + 1], cJ[KD_MAX + 1];
-  __shared__ __align__(16) uint8_t rI[CAP_I];
-  __shared__ __align__(16) float vI[CAP_I];
+  __shared__ int32_t tile[BAND_ROWS * TSTRIDE];     // fixed-point scores (x 2^S)
+  __shared__ uint64_t bJ[KW_MAX];
+  __shared__ uint16_t wpJ[KW_MAX];
+  __shared__ uint16_t cJ[KD_MAX + 1];
   __shared__ __align__(16) int2 eJ[CAP_J];          // {local row, value bits}
   __shared__ uint16_t s_nt[CAP_SI];                  // next J tile of each cursor
-  __shared__ uint2 wrec[NW][64];                     // per-wave common-feature records
-  __shared__ int32_t wincl[NW][64];                  // per-wave inclusive product prefix
-  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
-  __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
-  __shared__ apss_pair obuf[OBUF_CAP];                // emitted pairs waiting for a flush
+  __shared__ apss_pair obuf[OBUF_CAP];               // emitted pairs waiting for a flush
   __shared__ unsigned int obuf_n;
   __shared__ unsigned long long obuf_base;
+  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
+  __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
+  __shared__ float vI[CAP_I];                        // band entry values * 2^S, [q][tid]
   __shared__ int32_t J_first_s;
   __shared__ unsigned long long acc_products, acc_head, acc_cells;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
-  const int wave = tid / 64;
   const int KW = a.KW;
 
   const int32_t I = a.band_first + blockIdx.x;
@@ -126,22 +116,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int64_t i0 = a.I.tstart[I];
   const int nI = (int)(a.I.tstart[I + 1] - i0);
 
-  // ---- band set-up -------------------------------------------------------
+  // ---- band set-up: this thread's band entries live in registers ---------
+  // Entry k = tid + q * SCAN_THREADS of the band block: (row * TSTRIDE,
+  // bitmap index, value * 2^S).  The block is ordered by frequency rank, so
+  // the lanes of a wave hold entries of the same or similar features.
+  int32_t e_rf[I_PER_THREAD];  // row * TSTRIDE << 12 | bitmap index (or -1)
   {
-    for (int w = tid; w < KW; w += SCAN_THREADS) {
-      bI[w] = a.I.bmp[(int64_t)I * KW + w];
-      wpI[w] = a.I.wpre[(int64_t)I * KW + w];
-    }
-    const int64_t c0 = a.I.cbase[I];
-    const int ncs = (int)(a.I.cbase[I + 1] - c0);
-    for (int k = tid; k < ncs; k += SCAN_THREADS) cI[k] = (uint16_t)a.I.cst[c0 + k];
     const int64_t d0 = a.I.dbase[I];
-    const int nd4 = (int)((a.I.dbase[I + 1] - d0) >> 2);
-    for (int k = tid; k < nd4; k += SCAN_THREADS) {
-      reinterpret_cast<uint32_t*>(rI)[k] = reinterpret_cast<const uint32_t*>(a.I.drow + d0)[k];
-      float4 v = reinterpret_cast<const float4*>(a.I.dval + d0)[k];
-      v.x *= a.qscale; v.y *= a.qscale; v.z *= a.qscale; v.w *= a.qscale;  // exact: power of 2
-      reinterpret_cast<float4*>(vI)[k] = v;
+    const int nE = a.I.cst[a.I.cbase[I + 1] - 1];  // entries (end sentinel)
+#pragma unroll
+    for (int q = 0; q < I_PER_THREAD; ++q) {
+      const int k = tid + q * SCAN_THREADS;
+      const bool ok = k < nE;
+      e_rf[q] = ok ? (((int32_t)a.I.drow[d0 + k] * TSTRIDE) << 12) | (int32_t)a.I.dfeat[d0 + k] : -1;
+      vI[k] = ok ? a.I.dval[d0 + k] * a.qscale : 0.f;  // exact: power of 2
     }
   }
   for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
@@ -173,7 +161,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     a.scur[sb + k] = r.z;
     if (k < CAP_SI) s_nt[k] = (uint16_t)nt; else a.snext_spill[sb + k] = nt;
   }
-  unsigned long long my_products = 0, my_head = 0, my_cells = 0;
+  unsigned int my_products = 0, my_head = 0, my_cells = 0;
   __syncthreads();
   const int32_t J_first = J_first_s;
 
@@ -188,9 +176,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // ---- write the prefetched tile J into LDS --------------------------------
     if (tid < KW) { bJ[tid] = pf.bmp; wpJ[tid] = pf.wpre; }
 #pragma unroll
-    for (int i = 0; i < CST_PER_THREAD; ++i) {
-      const int k = tid + i * SCAN_THREADS;
-      if (k < pf.ncs) cJ[k] = (uint16_t)pf.cst[i];
+    for (int i = 0; i < CST_PAIRS; ++i) {
+      const int k = 2 * (tid + i * SCAN_THREADS);
+      if (k < pf.ncs) cJ[k] = (uint16_t)(pf.cst[i] & 0xffffu);
+      if (k + 1 < pf.ncs) cJ[k + 1] = (uint16_t)(pf.cst[i] >> 16);
     }
     if (tid < pf.nd4) {
       const float vv[4] = {pf.vals.x, pf.vals.y, pf.vals.z, pf.vals.w};
@@ -200,9 +189,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
     if (tid < TILE_ROWS) {
       const bool ok = tid < nJ;
-      normJ[tid] = ok ? pf.norm : 0.f;
-      hnJ[tid] = ok ? pf.hnorm : 0.f;
-      lenJ[tid] = ok ? pf.len : 0;
+      const int64_t r = j0 + tid;
+      normJ[tid] = ok ? a.J.norm[r] : 0.f;
+      hnJ[tid] = ok ? a.J.hnorm[r] : 0.f;
+      lenJ[tid] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
     }
 
     // ---- 1. cursor tier (score tile only) -----------------------------------
@@ -230,73 +220,33 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
     if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tid, pf);
 
-    // ---- 2. bitmap tier: wave-local merge path over the products -----------
-    for (int w = wave; w < KW; w += NW) {
-      const uint64_t wI = bI[w], wJ = bJ[w];
-      const uint64_t m = wI & wJ;
-      if (!m) continue;
-      int aI = 0, ca = 0, aJ = 0, cb = 0;
-      if ((m >> lane) & 1ull) {
-        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const int rkI = wpI[w] + __popcll(wI & below);
-        const int rkJ = wpJ[w] + __popcll(wJ & below);
-        aI = cI[rkI];
-        ca = cI[rkI + 1] - aI;
-        aJ = cJ[rkJ];
-        cb = cJ[rkJ + 1] - aJ;
-      }
-      int incl = ca * cb;
+    // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o);
-        if (lane >= o) incl += v;
+    for (int q = 0; q < I_PER_THREAD; ++q) {
+      int rf = e_rf[q];
+      // opaque per tile: keeps the compiler from hoisting per-entry masks and
+      // addresses out of the J loop (they would not fit in the register file)
+      asm volatile("" : "+v"(rf));
+      if (rf < 0) continue;
+      const int f = rf & 0xfff;
+      const int w = f >> 6, bit = f & 63;
+      const uint64_t wj = bJ[w];
+      if (!((wj >> bit) & 1ull)) continue;
+      const int rk = wpJ[w] + __popcll(wj & ((1ull << bit) - 1ull));
+      const int y0 = cJ[rk], y1 = cJ[rk + 1];
+      const int ib = rf >> 12;
+      const float u = vI[tid + q * SCAN_THREADS];
+      for (int y = y0; y < y1; ++y) {
+        const int2 e = eJ[y];
+        atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
       }
-      wrec[wave][lane] = make_uint2((uint32_t)aI | ((uint32_t)aJ << 16), (uint32_t)ca | ((uint32_t)cb << 8));
-      wincl[wave][lane] = incl;
-      __builtin_amdgcn_wave_barrier();
-      const int total = __shfl(incl, 63);
-      const int chunk = (total + 63) >> 6;
-      int q = lane * chunk;
-      const int qend = min(q + chunk, total);
-      if (q < qend) {
-        int lo = 0, hi = 63;  // first feature whose inclusive prefix exceeds q
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (wincl[wave][mid] > q) hi = mid; else lo = mid + 1;
-        }
-        int b = lo;
-        uint2 r = wrec[wave][b];
-        int fa = r.x & 0xffff, fb = r.x >> 16, fca = r.y & 0xff, fcb = r.y >> 8;
-        int loc = q - (wincl[wave][b] - fca * fcb);
-        int x = (int)((float)loc * __frcp_rn((float)fcb));
-        int y = loc - x * fcb;
-        if (y < 0) { --x; y += fcb; }
-        if (y >= fcb) { ++x; y -= fcb; }
-        int ib = rI[fa + x] * TSTRIDE;
-        float u = vI[fa + x];
-        for (;;) {
-          const int2 e = eJ[fb + y];
-          atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
-          if (++q == qend) break;
-          if (++y == fcb) {
-            y = 0;
-            if (++x == fca) {  // next common feature of this word
-              x = 0;
-              b = __ffsll((long long)(m & (~0ull << b) & ~(1ull << b))) - 1;
-              r = wrec[wave][b];
-              fa = r.x & 0xffff; fb = r.x >> 16; fca = r.y & 0xff; fcb = r.y >> 8;
-            }
-            ib = rI[fa + x] * TSTRIDE;
-            u = vI[fa + x];
-          }
-        }
-        my_products += (unsigned long long)(qend - lane * chunk);
-      }
-      __builtin_amdgcn_wave_barrier();
+      my_products += (unsigned)(y1 - y0);
     }
     __syncthreads();
+
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
+#pragma unroll 1
       for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
         const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
         if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
@@ -318,6 +268,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
 
     // ---- 4. threshold, compaction, reset -------------------------------------
+#pragma unroll 1
     for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
       const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
       int32_t* cell = &tile[il * TSTRIDE + jl];
@@ -384,9 +335,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
   }
 
-  atomicAdd(&acc_products, my_products);
-  atomicAdd(&acc_head, my_head);
-  atomicAdd(&acc_cells, my_cells);
+  atomicAdd(&acc_products, (unsigned long long)my_products);
+  atomicAdd(&acc_head, (unsigned long long)my_head);
+  atomicAdd(&acc_cells, (unsigned long long)my_cells);
   __syncthreads();
   if (tid == 0) {
     atomicAdd(&a.stats[0], acc_products);

@@ -138,3 +138,12 @@ def test_invalid_inputs_rejected():
     with pytest.raises(apss.ApssError) as e:
         apss.run(rp, badc, val, d, 0.5, 0)
     assert e.value.code == apss.APSS_E_INVALID
+
+
+def test_dense_output_overflows_buffers():
+    # a very low threshold makes most cells hits: exercises the per-workgroup
+    # output buffer overflow path and the HBM capacity regrowth
+    rp, col, val, d = apss.workload("small")
+    rp, col, val = subset(rp, col, val, 1200)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.01, 0)
+    assert len(exp) > 100_000
