@@ -24,12 +24,12 @@ namespace apss {
 constexpr int BAND_ROWS = 128;
 constexpr int TILE_ROWS = 64;
 constexpr int TSTRIDE = TILE_ROWS + 1;
-constexpr int SCAN_THREADS = 1024;           // band kernel workgroup (16 waves)
+constexpr int SCAN_THREADS = 512;            // band kernel workgroup (8 waves)
 constexpr int KD_MAX = 4096;                 // features in the bitmap tier
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
 constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
-constexpr int CAP_SI = 8192;                 // cursor-tier entries of a band kept in LDS
+constexpr int CAP_SI = 6144;                 // cursor-tier entries of a band kept in LDS
 constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
 
 // Feature classes in fmap[] (values >= 0 are bitmap-tier indices).
@@ -95,6 +95,7 @@ struct TileSide {
 struct CursorLists {
   const int64_t* ptr = nullptr;   // d+1
   const int2* ent = nullptr;
+  const uint16_t* tile = nullptr; // J tile of each entry's row
 };
 
 struct ScanArgs {

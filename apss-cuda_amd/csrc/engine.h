@@ -26,6 +26,7 @@ __global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
 __global__ void k_tile_compact(BuildSide s);
 __global__ void k_row2tile(const int64_t* tstart, int32_t* row2tile);
+__global__ void k_cursor_tiles(int64_t n, const int2* sent, const int32_t* row2tile, uint16_t* stile);
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
@@ -72,6 +73,7 @@ struct Tiers {
   int32_t* fmap = nullptr;       // d
   int64_t* sptr = nullptr;       // d+1
   int2* sent = nullptr;          // {row, value bits}
+  uint16_t* stile = nullptr;     // J tile of each entry's row (set per J tiling)
 };
 
 // One matrix side: CSR on the device and its norms.
@@ -139,6 +141,8 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 // corpus tiers, for tiles of at most max_rows rows and cap_d bitmap entries.
 TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
                     int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records);
+// J tile of every cursor-list entry for the J tiling `tiles`.
+void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles);
 // Band scan + rescore; output pairs stay on the device.
 ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
                             const Side& J, const TileSet& tiles, const Tiers& tr,

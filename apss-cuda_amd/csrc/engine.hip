@@ -412,6 +412,16 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   return ts;
 }
 
+void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles) {
+  if (tr.stile) ar.release(tr.stile);
+  tr.stile = ar.alloc<uint16_t>(tr.n_cursor_entries);
+  if (tr.n_cursor_entries > 0) {
+    hipLaunchKernelGGL(k_cursor_tiles, dim3(blocks_for(tr.n_cursor_entries, 256, 65536)), dim3(256),
+                       0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tr.stile);
+    APSS_HIP(hipGetLastError());
+  }
+}
+
 // ---------------------------------------------------------------------------
 // scan + rescore
 
@@ -450,6 +460,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.J = make_view(J, tiles);
     a.S.ptr = tr.sptr;
     a.S.ent = tr.sent;
+    a.S.tile = tr.stile;
     a.KD = tr.KD;
     a.KW = tr.KW;
     a.p = tr.p;
@@ -563,6 +574,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
   TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true);
   TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false);
+  set_cursor_tiles(ar, st, tr, tiles);
   out->ms_index = tm.stop_ms();
   RunConfig rc;
   rc.t = t;

@@ -385,6 +385,15 @@ __global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {
   if (threadIdx.x == 0) crow[wp[s.KW]] = prow[s.KD];
 }
 
+// J tile of the row of every cursor-list entry.
+__global__ __launch_bounds__(256) void k_cursor_tiles(int64_t n, const int2* __restrict__ sent,
+                                                      const int32_t* __restrict__ row2tile,
+                                                      uint16_t* __restrict__ stile) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    stile[e] = (uint16_t)row2tile[sent[e].x];
+}
+
 // Tile index of every row.
 __global__ __launch_bounds__(256) void k_row2tile(const int64_t* __restrict__ tstart,
                                                   int32_t* __restrict__ row2tile) {

@@ -55,15 +55,15 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 constexpr int OBUF_CAP = 512;               // LDS output buffer (pairs)
 constexpr int I_PER_THREAD = CAP_I / SCAN_THREADS;
 constexpr int CST_PAIRS = (KD_MAX + 2 + 2 * SCAN_THREADS - 1) / (2 * SCAN_THREADS);
-static_assert(CAP_J / 4 <= SCAN_THREADS, "one entry quad per thread");
+constexpr int QUADS_PER_THREAD = (CAP_J / 4 + SCAN_THREADS - 1) / SCAN_THREADS;
 static_assert(KD_MAX < 65536 && CAP_J < 65536, "16-bit segment starts");
 
 struct Prefetch {
   uint64_t bmp;
   uint32_t wpre;
   uint32_t cst[CST_PAIRS];  // two 16-bit segment starts per register
-  uint32_t rows;
-  float4 vals;
+  uint32_t rows[QUADS_PER_THREAD];
+  float4 vals[QUADS_PER_THREAD];
   int32_t ncs, nd4, nrows;
 };
 
@@ -85,9 +85,13 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int 
     uint32_t hi = k + 1 < pf.ncs ? (uint32_t)S.cst[c0 + k + 1] : 0u;
     pf.cst[i] = lo | (hi << 16);
   }
-  if (tid < pf.nd4) {
-    pf.rows = reinterpret_cast<const uint32_t*>(S.drow + d0)[tid];
-    pf.vals = reinterpret_cast<const float4*>(S.dval + d0)[tid];
+#pragma unroll
+  for (int i = 0; i < QUADS_PER_THREAD; ++i) {
+    const int k = tid + i * SCAN_THREADS;
+    if (k < pf.nd4) {
+      pf.rows[i] = reinterpret_cast<const uint32_t*>(S.drow + d0)[k];
+      pf.vals[i] = reinterpret_cast<const float4*>(S.dval + d0)[k];
+    }
   }
 }
 
@@ -98,6 +102,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ uint16_t cJ[KD_MAX + 1];
   __shared__ __align__(16) int2 eJ[CAP_J];          // {local row, value bits}
   __shared__ uint16_t s_nt[CAP_SI];                  // next J tile of each cursor
+  __shared__ int32_t s_cur[CAP_SI];                  // its position in the cursor list
   __shared__ apss_pair obuf[OBUF_CAP];               // emitted pairs waiting for a flush
   __shared__ unsigned int obuf_n;
   __shared__ unsigned long long obuf_base;
@@ -157,9 +162,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int nS = (int)(a.I.sbase[I + 1] - sb);
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     int4 r = a.I.srec[sb + k];
-    int32_t nt = r.z < r.w ? a.J.row2tile[a.S.ent[r.z].x] : 0xffff;
-    a.scur[sb + k] = r.z;
-    if (k < CAP_SI) s_nt[k] = (uint16_t)nt; else a.snext_spill[sb + k] = nt;
+    int32_t nt = r.z < r.w ? (int32_t)a.S.tile[r.z] : 0xffff;
+    if (k < CAP_SI) { s_nt[k] = (uint16_t)nt; s_cur[k] = r.z; }
+    else { a.snext_spill[sb + k] = nt; a.scur[sb + k] = r.z; }
   }
   unsigned int my_products = 0, my_head = 0, my_cells = 0;
   __syncthreads();
@@ -181,11 +186,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if (k < pf.ncs) cJ[k] = (uint16_t)(pf.cst[i] & 0xffffu);
       if (k + 1 < pf.ncs) cJ[k + 1] = (uint16_t)(pf.cst[i] >> 16);
     }
-    if (tid < pf.nd4) {
-      const float vv[4] = {pf.vals.x, pf.vals.y, pf.vals.z, pf.vals.w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        eJ[4 * tid + q] = make_int2((int)((pf.rows >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
+    for (int i = 0; i < QUADS_PER_THREAD; ++i) {
+      const int k = tid + i * SCAN_THREADS;
+      if (k < pf.nd4) {
+        const float vv[4] = {pf.vals[i].x, pf.vals[i].y, pf.vals[i].z, pf.vals[i].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
+      }
     }
     if (tid < TILE_ROWS) {
       const bool ok = tid < nJ;
@@ -195,52 +204,91 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       lenJ[tid] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
     }
 
-    // ---- 1. cursor tier (score tile only) -----------------------------------
+    __syncthreads();
+
+    // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
+    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tid, pf);
+
+    // ---- 1. cursor tier, first half: find this thread's first cursor in J
+    // and issue its loads; they complete behind the bitmap-tier products ----
+    int hk = -1;
+    int4 hrec = make_int4(0, 0, 0, 0);
+    int2 hent = make_int2(0, 0);
     for (int k = tid; k < nS; k += SCAN_THREADS) {
       const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
-      if (nt != J) continue;
-      const int4 r = a.I.srec[sb + k];
-      int32_t cur = a.scur[sb + k];
+      if (nt == J) {
+        hk = k;
+        const int32_t cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
+        hrec = a.I.srec[sb + k];
+        hent = a.S.ent[cur];
+        break;
+      }
+    }
+
+    // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
+    // Two passes of I_PER_THREAD/2 entries: the LDS lookups of a pass are
+    // issued together, then its segments are walked.
+#pragma unroll
+    for (int h = 0; h < I_PER_THREAD; h += I_PER_THREAD / 2) {
+      int y0[I_PER_THREAD / 2], y1[I_PER_THREAD / 2];
+#pragma unroll
+      for (int q = 0; q < I_PER_THREAD / 2; ++q) {
+        int rf = e_rf[h + q];
+        asm volatile("" : "+v"(rf));  // no hoisting of per-entry values out of the J loop
+        y0[q] = 0; y1[q] = 0;
+        if (rf >= 0) {
+          const int f = rf & 0xfff;
+          const int w = f >> 6, bit = f & 63;
+          const uint64_t wj = bJ[w];
+          if ((wj >> bit) & 1ull) {
+            const int rk = wpJ[w] + __popcll(wj & ((1ull << bit) - 1ull));
+            y0[q] = cJ[rk];
+            y1[q] = cJ[rk + 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < I_PER_THREAD / 2; ++q) {
+        if (y1[q] <= y0[q]) continue;
+        const int ib = e_rf[h + q] >> 12;
+        const float u = vI[tid + (h + q) * SCAN_THREADS];
+        for (int y = y0[q]; y < y1[q]; ++y) {
+          const int2 e = eJ[y];
+          atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
+        }
+        my_products += (unsigned)(y1[q] - y0[q]);
+      }
+    }
+
+    // ---- 1. cursor tier, second half: walk the cursors that point into J ---
+    for (int k = hk; k >= 0 && k < nS; k += SCAN_THREADS) {
+      int32_t cur;
+      int4 r;
+      int2 e;
+      if (k == hk) {  // loads issued before the bitmap phase
+        cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
+        r = hrec;
+        e = hent;
+      } else {
+        const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
+        if (nt != J) continue;
+        cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
+        r = a.I.srec[sb + k];
+        e = a.S.ent[cur];
+      }
       const float x = __int_as_float(r.y) * a.qscale;
       const int rowbase = r.x * TSTRIDE - (int)j0;
-      int2 e = a.S.ent[cur];
       int32_t ntn = 0xffff;
       for (;;) {
         atomicAdd(&tile[rowbase + e.x], __float2int_rn(x * __int_as_float(e.y)));
         ++my_products;
         if (++cur >= r.w) break;
         e = a.S.ent[cur];
-        if (e.x >= j1) { ntn = a.J.row2tile[e.x]; break; }
+        const int32_t tn = (int32_t)a.S.tile[cur];
+        if (tn != J) { ntn = tn; break; }
       }
-      a.scur[sb + k] = cur;
-      if (k < CAP_SI) s_nt[k] = (uint16_t)ntn; else a.snext_spill[sb + k] = ntn;
-    }
-    __syncthreads();
-
-    // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
-    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tid, pf);
-
-    // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
-#pragma unroll
-    for (int q = 0; q < I_PER_THREAD; ++q) {
-      int rf = e_rf[q];
-      // opaque per tile: keeps the compiler from hoisting per-entry masks and
-      // addresses out of the J loop (they would not fit in the register file)
-      asm volatile("" : "+v"(rf));
-      if (rf < 0) continue;
-      const int f = rf & 0xfff;
-      const int w = f >> 6, bit = f & 63;
-      const uint64_t wj = bJ[w];
-      if (!((wj >> bit) & 1ull)) continue;
-      const int rk = wpJ[w] + __popcll(wj & ((1ull << bit) - 1ull));
-      const int y0 = cJ[rk], y1 = cJ[rk + 1];
-      const int ib = rf >> 12;
-      const float u = vI[tid + q * SCAN_THREADS];
-      for (int y = y0; y < y1; ++y) {
-        const int2 e = eJ[y];
-        atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
-      }
-      my_products += (unsigned)(y1 - y0);
+      if (k < CAP_SI) { s_nt[k] = (uint16_t)ntn; s_cur[k] = cur; }
+      else { a.snext_spill[sb + k] = ntn; a.scur[sb + k] = cur; }
     }
     __syncthreads();
 
@@ -268,17 +316,31 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
 
     // ---- 4. threshold, compaction, reset -------------------------------------
+    // Pass 1 reads this thread's cells together and marks the ones that can
+    // reach t; pass 2 (rolled) settles candidates and clears every cell.
+    constexpr int CELLS_PER_THREAD = BAND_ROWS * TILE_ROWS / SCAN_THREADS;
+    unsigned cand = 0;
+#pragma unroll
+    for (int m = 0; m < CELLS_PER_THREAD; ++m) {
+      const int c = tid + m * SCAN_THREADS;
+      const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
+      const int32_t q = tile[il * TSTRIDE + jl];
+      const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
+      my_cells += valid ? 1u : 0u;
+      cand |= (valid && q >= a.q_lo) ? (1u << m) : 0u;
+    }
 #pragma unroll 1
-    for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
+    for (int m = 0; m < CELLS_PER_THREAD; ++m) {
+      const int c = tid + m * SCAN_THREADS;
       const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
       int32_t* cell = &tile[il * TSTRIDE + jl];
+      const bool is_cand = (cand >> m) & 1u;
+      if (!__ballot(is_cand)) { *cell = 0; continue; }
       const int32_t q = *cell;
       *cell = 0;
-      const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
-      my_cells += valid ? 1 : 0;
       bool emit = false, to_band = false;
       double sd = 0.0;
-      if (valid && q >= a.q_lo) {
+      if (is_cand) {
         const int k = lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl];
         const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jl] : 0.0;
         const double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
