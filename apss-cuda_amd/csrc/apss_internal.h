@@ -24,7 +24,7 @@ namespace apss {
 constexpr int BAND_ROWS = 128;
 constexpr int TILE_ROWS = 64;
 constexpr int TSTRIDE = TILE_ROWS + 1;
-constexpr int SCAN_THREADS = 512;            // band kernel workgroup (8 waves)
+constexpr int SCAN_THREADS = 1024;           // band kernel workgroup (16 waves)
 constexpr int KD_MAX = 4096;                 // features in the bitmap tier
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band

@@ -174,21 +174,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   if (J_first < a.J.T) prefetch_tile(a.J, J_first, KW, tid, pf);
 
   for (int32_t J = J_first; J < a.J.T; ++J) {
+    // per-thread addresses are recomputed from an opaque thread index in every
+    // pass instead of being kept live across the loop (register budget)
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
     const int64_t j0 = a.J.tstart[J];
     const int nJ = pf.nrows;
     const int64_t j1 = j0 + nJ;
 
     // ---- write the prefetched tile J into LDS --------------------------------
-    if (tid < KW) { bJ[tid] = pf.bmp; wpJ[tid] = pf.wpre; }
+    if (tl < KW) { bJ[tl] = pf.bmp; wpJ[tl] = pf.wpre; }
 #pragma unroll
     for (int i = 0; i < CST_PAIRS; ++i) {
-      const int k = 2 * (tid + i * SCAN_THREADS);
+      const int k = 2 * (tl + i * SCAN_THREADS);
       if (k < pf.ncs) cJ[k] = (uint16_t)(pf.cst[i] & 0xffffu);
       if (k + 1 < pf.ncs) cJ[k + 1] = (uint16_t)(pf.cst[i] >> 16);
     }
 #pragma unroll
     for (int i = 0; i < QUADS_PER_THREAD; ++i) {
-      const int k = tid + i * SCAN_THREADS;
+      const int k = tl + i * SCAN_THREADS;
       if (k < pf.nd4) {
         const float vv[4] = {pf.vals[i].x, pf.vals[i].y, pf.vals[i].z, pf.vals[i].w};
 #pragma unroll
@@ -196,25 +200,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
       }
     }
-    if (tid < TILE_ROWS) {
-      const bool ok = tid < nJ;
-      const int64_t r = j0 + tid;
-      normJ[tid] = ok ? a.J.norm[r] : 0.f;
-      hnJ[tid] = ok ? a.J.hnorm[r] : 0.f;
-      lenJ[tid] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
+    if (tl < TILE_ROWS) {
+      const bool ok = tl < nJ;
+      const int64_t r = j0 + tl;
+      normJ[tl] = ok ? a.J.norm[r] : 0.f;
+      hnJ[tl] = ok ? a.J.hnorm[r] : 0.f;
+      lenJ[tl] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
     }
 
     __syncthreads();
 
     // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
-    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tid, pf);
+    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tl, pf);
 
     // ---- 1. cursor tier, first half: find this thread's first cursor in J
     // and issue its loads; they complete behind the bitmap-tier products ----
     int hk = -1;
     int4 hrec = make_int4(0, 0, 0, 0);
     int2 hent = make_int2(0, 0);
-    for (int k = tid; k < nS; k += SCAN_THREADS) {
+    for (int k = tl; k < nS; k += SCAN_THREADS) {
       const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
       if (nt == J) {
         hk = k;
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int q = 0; q < I_PER_THREAD / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
         const int ib = e_rf[h + q] >> 12;
-        const float u = vI[tid + (h + q) * SCAN_THREADS];
+        const float u = vI[tl + (h + q) * SCAN_THREADS];
         for (int y = y0[q]; y < y1[q]; ++y) {
           const int2 e = eJ[y];
           atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
 #pragma unroll 1
-      for (int c = tid; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
+      for (int c = tl; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
         const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
         if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
         int32_t* cell = &tile[il * TSTRIDE + jl];
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     unsigned cand = 0;
 #pragma unroll
     for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-      const int c = tid + m * SCAN_THREADS;
+      const int c = tl + m * SCAN_THREADS;
       const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
       const int32_t q = tile[il * TSTRIDE + jl];
       const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
 #pragma unroll 1
     for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-      const int c = tid + m * SCAN_THREADS;
+      const int c = tl + m * SCAN_THREADS;
       const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
       int32_t* cell = &tile[il * TSTRIDE + jl];
       const bool is_cand = (cand >> m) & 1u;
@@ -386,13 +390,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const unsigned int nbuf = obuf_n;
     if (nbuf >= OBUF_CAP / 2 || (J + 1 == a.J.T && nbuf > 0)) {
       const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
-      if (tid == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
+      if (tl == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
       __syncthreads();
       const unsigned long long ob = obuf_base;
-      for (unsigned int k = tid; k < nf; k += SCAN_THREADS)
+      for (unsigned int k = tl; k < nf; k += SCAN_THREADS)
         if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
       __syncthreads();
-      if (tid == 0) obuf_n = 0;
+      if (tl == 0) obuf_n = 0;
       __syncthreads();
     }
   }
