@@ -43,7 +43,8 @@ class Opts(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("count", c_int64), ("pairs", c_void_p), ("on_device", c_int32),
                 ("p_used", c_int32), ("cells", c_int64), ("band", c_int64),
-                ("products", c_int64), ("head_pairs", c_int64),
+                ("products", c_int64), ("head_pairs", c_int64), ("list_visits", c_int64),
+                ("cursor_products", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
                 ("internal_", c_void_p)]
@@ -201,3 +202,92 @@ def run(rp, col, val, d, t, p=0, **opts):
 
 def sort_pairs(pairs):
     return pairs[np.lexsort((pairs["j"], pairs["i"]))]
+
+
+# ---------------------------------------------------------------------------
+# minimal HIP runtime access (device buffers, streams, events) for callers that
+# hand device-resident CSR to the library
+
+class Hip:
+    """ctypes view of the few libamdhip64 calls the benchmarks need."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipGetErrorString.restype = ctypes.c_char_p
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {self.L.hipGetErrorString(rc).decode()}")
+
+    def malloc(self, nbytes):
+        p = c_void_p()
+        self._ok(self.L.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(int(nbytes), 1))), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self._ok(self.L.hipFree(c_void_p(p)), "hipFree")
+
+    def h2d(self, dst, arr):
+        a = np.ascontiguousarray(arr)
+        self._ok(self.L.hipMemcpy(c_void_p(dst), c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1),
+                 "hipMemcpy H2D")
+
+    def d2h(self, arr, src):
+        self._ok(self.L.hipMemcpy(c_void_p(arr.ctypes.data), c_void_p(src), ctypes.c_size_t(arr.nbytes), 2),
+                 "hipMemcpy D2H")
+
+    def sync(self):
+        self._ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def stream(self):
+        s = c_void_p()
+        self._ok(self.L.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def event(self):
+        e = c_void_p()
+        self._ok(self.L.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e.value
+
+    def record(self, ev, stream):
+        self._ok(self.L.hipEventRecord(c_void_p(ev), c_void_p(stream)), "hipEventRecord")
+
+    def elapsed_ms(self, e0, e1):
+        self._ok(self.L.hipEventSynchronize(c_void_p(e1)), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        self._ok(self.L.hipEventElapsedTime(ctypes.byref(ms), c_void_p(e0), c_void_p(e1)), "hipEventElapsedTime")
+        return float(ms.value)
+
+
+class DeviceCSR:
+    """A CSR matrix copied to device memory (freed with .release())."""
+
+    def __init__(self, hip, rp, col, val, d):
+        self.hip = hip
+        self.n, self.nnz, self.d = len(rp) - 1, len(col), d
+        self.rp = hip.malloc(rp.nbytes)
+        self.col = hip.malloc(col.nbytes)
+        self.val = hip.malloc(val.nbytes)
+        hip.h2d(self.rp, rp)
+        if len(col):
+            hip.h2d(self.col, col)
+            hip.h2d(self.val, val)
+
+    def csr(self):
+        return CSR(n_rows=self.n, n_cols=self.d, nnz=self.nnz, row_ptr=self.rp, col=self.col, val=self.val,
+                   on_device=1)
+
+    def release(self):
+        for p in (self.rp, self.col, self.val):
+            self.hip.free(p)
+
+
+def run_csr(X, t, p=0, **opts):
+    """apss_run on a prepared CSR struct; returns the Result (caller frees it)."""
+    L = lib()
+    o = make_opts(**opts)
+    r = Result()
+    rc = L.apss_run(ctypes.byref(X), float(t), int(p), ctypes.byref(o), ctypes.byref(r))
+    if rc != APSS_OK:
+        raise ApssError(rc, L.apss_last_error().decode())
+    return r

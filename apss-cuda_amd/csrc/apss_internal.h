@@ -123,7 +123,7 @@ struct ScanArgs {
   int32_t band_first;        // first band (tile row of I) of this launch
   int32_t band_count;
   int32_t pad2_;
-  unsigned long long* stats; // [0] products, [1] head pairs, [2] cells
+  unsigned long long* stats; // [0] products [1] head pairs [2] cells [3] list visits [4] cursor products
   int32_t* scur;             // cursor position of every cursor record (I side)
   int32_t* snext_spill;      // next tile of records beyond CAP_SI
 };

@@ -129,7 +129,7 @@ struct RunConfig {
 struct ScanOutput {
   apss_pair* pairs = nullptr;   // device, owned by the arena
   int64_t count = 0;
-  int64_t band = 0, products = 0, head_pairs = 0, cells = 0;
+  int64_t band = 0, products = 0, head_pairs = 0, cells = 0, visits = 0, cursor = 0;
   double ms_scan = 0, ms_rescore = 0;
 };
 

@@ -505,6 +505,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.products = (int64_t)h[2];
     so.head_pairs = (int64_t)h[3];
     so.cells = (int64_t)h[4];
+    so.visits = (int64_t)h[5];
+    so.cursor = (int64_t)h[6];
     so.band = nband;
     if (nout + nband > cap) {  // room for every band pair before the rescore
       apss_pair* bigger = ar.alloc<apss_pair>(nout + nband);
@@ -590,6 +592,8 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->band = so.band;
   out->products = so.products;
   out->head_pairs = so.head_pairs;
+  out->list_visits = so.visits;
+  out->cursor_products = so.cursor;
   tm.start();
   if (o->output_on_device && !o->sort_output) {
     out->pairs = (apss_pair*)ar.detach(so.pairs);

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
   __shared__ float vI[CAP_I];                        // band entry values * 2^S, [q][tid]
   __shared__ int32_t J_first_s;
-  __shared__ unsigned long long acc_products, acc_head, acc_cells;
+  __shared__ unsigned long long acc_products, acc_head, acc_cells, acc_visits, acc_cursor;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
@@ -133,7 +133,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     for (int q = 0; q < I_PER_THREAD; ++q) {
       const int k = tid + q * SCAN_THREADS;
       const bool ok = k < nE;
-      e_rf[q] = ok ? (((int32_t)a.I.drow[d0 + k] * TSTRIDE) << 12) | (int32_t)a.I.dfeat[d0 + k] : -1;
+      // bit 26 marks the first entry of a feature's band segment (it counts
+      // the tile segment once in the list-visit statistics)
+      const int32_t f = ok ? (int32_t)a.I.dfeat[d0 + k] : 0;
+      const bool seg_head = ok && (k == 0 || (int32_t)a.I.dfeat[d0 + k - 1] != f);
+      e_rf[q] = ok ? ((((int32_t)a.I.drow[d0 + k] * TSTRIDE) << 12) | f | (seg_head ? (1 << 26) : 0)) : -1;
       vI[k] = ok ? a.I.dval[d0 + k] * a.qscale : 0.f;  // exact: power of 2
     }
   }
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
   }
   if (tid == 0) {
-    acc_products = 0; acc_head = 0; acc_cells = 0; obuf_n = 0;
+    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
     int32_t jf = 0;
     if (a.self) {  // first tile whose rows end after the band's first row
       int32_t lo = 0, hi = a.J.T;
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     if (k < CAP_SI) { s_nt[k] = (uint16_t)nt; s_cur[k] = r.z; }
     else { a.snext_spill[sb + k] = nt; a.scur[sb + k] = r.z; }
   }
-  unsigned int my_products = 0, my_head = 0, my_cells = 0;
+  unsigned int my_products = 0, my_head = 0, my_cells = 0, my_visits = 0, my_cursor = 0;
   __syncthreads();
   const int32_t J_first = J_first_s;
 
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         asm volatile("" : "+v"(rf));  // no hoisting of per-entry values out of the J loop
         y0[q] = 0; y1[q] = 0;
         if (rf >= 0) {
-          const int f = rf & 0xfff;
+          const int f = rf & 0xfff;  // bitmap index (bits 0..11)
           const int w = f >> 6, bit = f & 63;
           const uint64_t wj = bJ[w];
           if ((wj >> bit) & 1ull) {
@@ -254,7 +258,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
 #pragma unroll
       for (int q = 0; q < I_PER_THREAD / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
-        const int ib = e_rf[h + q] >> 12;
+        my_visits += 1u + (((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u);
+        const int ib = (e_rf[h + q] >> 12) & 0x3fff;
         const float u = vI[tl + (h + q) * SCAN_THREADS];
         for (int y = y0[q]; y < y1[q]; ++y) {
           const int2 e = eJ[y];
@@ -286,6 +291,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (;;) {
         atomicAdd(&tile[rowbase + e.x], __float2int_rn(x * __int_as_float(e.y)));
         ++my_products;
+        ++my_cursor;
         if (++cur >= r.w) break;
         e = a.S.ent[cur];
         const int32_t tn = (int32_t)a.S.tile[cur];
@@ -404,11 +410,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   atomicAdd(&acc_products, (unsigned long long)my_products);
   atomicAdd(&acc_head, (unsigned long long)my_head);
   atomicAdd(&acc_cells, (unsigned long long)my_cells);
+  atomicAdd(&acc_visits, (unsigned long long)my_visits);
+  atomicAdd(&acc_cursor, (unsigned long long)my_cursor);
   __syncthreads();
   if (tid == 0) {
     atomicAdd(&a.stats[0], acc_products);
     atomicAdd(&a.stats[1], acc_head);
     atomicAdd(&a.stats[2], acc_cells);
+    atomicAdd(&a.stats[3], acc_visits);
+    atomicAdd(&a.stats[4], acc_cursor);
   }
 }
 

@@ -96,6 +96,8 @@ typedef struct {
   int64_t band;            /* pairs sent to the FP64 rescore */
   int64_t products;        /* tail partial products accumulated */
   int64_t head_pairs;      /* pairs that ran the head forward-list dot */
+  int64_t list_visits;     /* inverted-list entries visited by tile pairs (both sides) */
+  int64_t cursor_products; /* products of the short-list (cursor) tier */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   void* internal_;

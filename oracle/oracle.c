@@ -257,6 +257,7 @@ int oracle_hybrid_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* c
   pbuf* bufs = (pbuf*)calloc((size_t)(nb ? nb : 1), sizeof(pbuf));
   int64_t cand = 0, surv = 0, gathers = 0;
   int err = bufs ? 0 : -1;
+  const double t_search0 = omp_get_wtime();
 #pragma omp parallel reduction(+ : cand, surv, gathers)
   {
     float* acc = (float*)calloc((size_t)(n ? n : 1), sizeof(float));
@@ -301,8 +302,12 @@ int oracle_hybrid_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* c
     }
     free(acc); free(head);
   }
+  const double t_search = omp_get_wtime() - t_search0;
   free(cptr); free(crow); free(cval); free(hn);
-  if (st) { st->candidates = cand; st->survivors = surv; st->head_gathers = gathers; }
+  if (st) {
+    st->candidates = cand; st->survivors = surv; st->head_gathers = gathers;
+    st->search_seconds = t_search;
+  }
   if (!bufs) return -1;
   int rc = gather(bufs, nb, out);
   return err ? -1 : rc;

@@ -18,6 +18,7 @@ typedef struct {
   int64_t candidates;   /* pairs the prefix-norm filter looked at */
   int64_t survivors;    /* pairs that passed it */
   int64_t head_gathers; /* head entries walked by the forward-list dot */
+  double search_seconds; /* wall time of the search loop (index build excluded) */
 } oracle_hybrid_stats;
 
 int oracle_transpose(int64_t n, int64_t d, const int64_t* rp, const int32_t* col, const float* val,

@@ -19,7 +19,8 @@ class _Pairs(ctypes.Structure):
 
 
 class _HStats(ctypes.Structure):
-    _fields_ = [("candidates", c_int64), ("survivors", c_int64), ("head_gathers", c_int64)]
+    _fields_ = [("candidates", c_int64), ("survivors", c_int64), ("head_gathers", c_int64),
+                ("search_seconds", c_double)]
 
 
 PAIR64 = np.dtype([("i", "<u4"), ("j", "<u4"), ("score", "<f8")])
@@ -97,7 +98,8 @@ def hybrid_fp32(rp, col, val, d, t, p, rows=None):
     if lib().oracle_hybrid_fp32(len(rp) - 1, d, prp, pc, pv, float(t), int(p), prows, nr,
                                 ctypes.byref(out), ctypes.byref(st)) != 0:
         raise MemoryError("oracle_hybrid_fp32 failed")
-    return _take(out), dict(candidates=st.candidates, survivors=st.survivors, head_gathers=st.head_gathers)
+    return _take(out), dict(candidates=st.candidates, survivors=st.survivors, head_gathers=st.head_gathers,
+                            search_seconds=st.search_seconds)
 
 
 def transpose(rp, col, val, d, p=0):
