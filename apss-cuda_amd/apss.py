@@ -291,3 +291,166 @@ def run_csr(X, t, p=0, **opts):
     if rc != APSS_OK:
         raise ApssError(rc, L.apss_last_error().decode())
     return r
+
+
+# ---------------------------------------------------------------------------
+# batched, index, server and compat entry points
+
+def _bind_more(L):
+    if getattr(L, "_more_bound", False):
+        return L
+    L.apss_run_batched.argtypes = [ctypes.c_void_p, c_int32, ctypes.c_void_p, c_int32, POINTER(Opts), c_void_p]
+    L.apss_run_batched.restype = c_int32
+    L.apss_index_create.argtypes = [POINTER(CSR), c_int32, POINTER(Opts), POINTER(c_void_p)]
+    L.apss_index_create.restype = c_int32
+    L.apss_index_insert.argtypes = [c_void_p, POINTER(CSR)]
+    L.apss_index_insert.restype = c_int32
+    L.apss_index_query.argtypes = [c_void_p, POINTER(CSR), c_double, POINTER(Result)]
+    L.apss_index_query.restype = c_int32
+    L.apss_index_rows.argtypes = [c_void_p]
+    L.apss_index_rows.restype = c_int64
+    L.apss_index_destroy.argtypes = [c_void_p]
+    L.apss_server_create.argtypes = [c_void_p, c_int32, c_int64, c_int32, POINTER(c_void_p)]
+    L.apss_server_create.restype = c_int32
+    L.apss_server_submit.argtypes = [c_void_p, POINTER(CSR), c_double]
+    L.apss_server_submit.restype = c_int64
+    L.apss_server_wait.argtypes = [c_void_p, c_int64, POINTER(Result), POINTER(c_double)]
+    L.apss_server_wait.restype = c_int32
+    L.apss_server_cancel.argtypes = [c_void_p, c_int64]
+    L.apss_server_cancel.restype = c_int32
+    L.apss_server_destroy.argtypes = [c_void_p]
+    L.apss_compat_last_count.restype = c_int64
+    L._more_bound = True
+    return L
+
+
+def _check(rc):
+    if rc != APSS_OK:
+        raise ApssError(rc, lib().apss_last_error().decode())
+
+
+def _take_result(r):
+    pairs = result_pairs(r)
+    info = {f: getattr(r, f) for f, _ in Result._fields_ if f not in ("pairs", "internal_")}
+    lib().apss_result_free(ctypes.byref(r))
+    return pairs, info
+
+
+def run_batched(instances, ts, p=0, **opts):
+    """instances: list of (rp, col, val, d); ts: thresholds.  Returns list of (pairs, info)."""
+    L = _bind_more(lib())
+    k = len(instances)
+    keep = [(np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(c, np.int32),
+             np.ascontiguousarray(v, np.float32), d) for rp, c, v, d in instances]
+    csrs = (CSR * max(k, 1))(*[make_csr(*x) for x in keep])
+    ptrs = (ctypes.c_void_p * max(k, 1))(*[ctypes.addressof(csrs[i]) for i in range(k)])
+    tarr = (c_double * max(k, 1))(*[float(x) for x in ts])
+    res = (Result * max(k, 1))()
+    o = make_opts(**opts)
+    _check(L.apss_run_batched(ptrs, k, tarr, int(p), ctypes.byref(o), res))
+    return [_take_result(res[i]) for i in range(k)]
+
+
+class Index:
+    """Corpus index resident in device memory (apss_index_*)."""
+
+    def __init__(self, rp, col, val, d, p=0, **opts):
+        self.L = _bind_more(lib())
+        self.d = d
+        X = make_csr(rp, col, val, d)
+        o = make_opts(**opts)
+        h = c_void_p()
+        _check(self.L.apss_index_create(ctypes.byref(X), int(p), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+
+    def query(self, rp, col, val, t):
+        X = make_csr(rp, col, val, self.d)
+        r = Result()
+        _check(self.L.apss_index_query(self.h, ctypes.byref(X), float(t), ctypes.byref(r)))
+        return _take_result(r)
+
+    def insert(self, rp, col, val):
+        X = make_csr(rp, col, val, self.d)
+        _check(self.L.apss_index_insert(self.h, ctypes.byref(X)))
+
+    def rows(self):
+        return self.L.apss_index_rows(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.apss_index_destroy(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching server over an Index (apss_server_*)."""
+
+    def __init__(self, index, slots=2, mem_budget=0, max_queue=0):
+        self.L = _bind_more(lib())
+        self.index = index
+        h = c_void_p()
+        _check(self.L.apss_server_create(index.h, int(slots), int(mem_budget), int(max_queue), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, rp, col, val, t):
+        X = make_csr(rp, col, val, self.index.d)
+        tk = self.L.apss_server_submit(self.h, ctypes.byref(X), float(t))
+        if tk < 0:
+            raise ApssError(int(tk), lib().apss_last_error().decode())
+        return tk
+
+    def wait(self, ticket):
+        r = Result()
+        lat = c_double()
+        rc = self.L.apss_server_wait(self.h, int(ticket), ctypes.byref(r), ctypes.byref(lat))
+        _check(rc)
+        pairs, info = _take_result(r)
+        return pairs, lat.value
+
+    def cancel(self, ticket):
+        return self.L.apss_server_cancel(self.h, int(ticket))
+
+    def close(self):
+        if self.h:
+            self.L.apss_server_destroy(self.h)
+            self.h = None
+
+
+def sell32_partition(rp, col, val):
+    """SELL-32 arrays (dim, feature, tile_start) of a CSR block, as main.cu builds
+    them per partition (main.cu:286-339): slices of 32 rows stored column-major,
+    width = the largest row of the slice, padding id -1."""
+    n = len(rp) - 1
+    tiles = (n + 31) // 32
+    starts = np.zeros(tiles + 1, np.int32)
+    lens = np.diff(rp)
+    for tt in range(tiles):
+        w = int(lens[tt * 32:(tt + 1) * 32].max()) if n else 0
+        starts[tt + 1] = starts[tt] + 32 * w
+    dim = np.full(max(int(starts[-1]), 1), -1, np.int32)
+    feat = np.zeros(max(int(starts[-1]), 1), np.float32)
+    for r in range(n):
+        base = starts[r // 32] + r % 32
+        k = int(lens[r])
+        dim[base + 32 * np.arange(k)] = col[rp[r]:rp[r + 1]]
+        feat[base + 32 * np.arange(k)] = val[rp[r]:rp[r + 1]]
+    return dim, feat, starts
+
+
+def similarity_join_device(parts):
+    """Call the reference-signature shim with 8 partitions (list of (rp, col, val))."""
+    L = _bind_more(lib())
+    assert len(parts) == 8
+    sells = [sell32_partition(*p) for p in parts]
+    args = []
+    for dim, feat, starts in sells:
+        args += [dim.ctypes.data_as(c_void_p), feat.ctypes.data_as(c_void_p), starts.ctypes.data_as(c_void_p)]
+    norms = [np.zeros(max(len(p[0]) - 1, 1), np.float32) for p in parts]
+    args += [x.ctypes.data_as(c_void_p) for x in norms]
+    for (rp, col, val), (dim, feat, starts) in zip(parts, sells):
+        args += [ctypes.c_int(len(rp) - 1), ctypes.c_int(len(dim)), ctypes.c_int(len(starts) - 1)]
+    args += [c_void_p(None), c_void_p(None), c_void_p(None)]
+    L.similarity_join_device.restype = None
+    L.similarity_join_device(*args)
+    return L.apss_compat_last_count()

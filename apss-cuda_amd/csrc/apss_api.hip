@@ -79,6 +79,11 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   return APSS_OK;
 }
 
+int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
+                     const apss_opts* o, apss_result* out) {
+  return guarded([&] { return run_batched(X, count, t, p, o, out); });
+}
+
 void apss_result_free(apss_result* r) {
   if (!r) return;
   if (r->pairs) {

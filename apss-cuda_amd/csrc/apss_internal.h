@@ -124,6 +124,9 @@ struct ScanArgs {
   int32_t band_count;
   int32_t pad2_;
   unsigned long long* stats; // [0] products [1] head pairs [2] cells [3] list visits [4] cursor products
+  const int32_t* band_jend;  // per band: one past its last partner tile (NULL: all)
+  const double* band_t;      // per band: threshold (NULL: t)
+  double eps_max;            // largest cell error bound (for per-band fast rejects)
   int32_t* scur;             // cursor position of every cursor record (I side)
   int32_t* snext_spill;      // next tile of records beyond CAP_SI
 };

@@ -104,6 +104,7 @@ struct TileSet {
   int64_t* sbase = nullptr;
   int4* srec = nullptr;
   int64_t nD = 0, nS = 0;
+  std::vector<int64_t> cuts;     // host copy of tstart
 };
 
 TileSide make_view(const Side& s, const TileSet& ts);
@@ -124,6 +125,9 @@ struct RunConfig {
   int32_t rescore_all = 0;
   int32_t kd_cap = 0;
   int64_t i_off = 0, j_off = 0;
+  const int32_t* band_jend = nullptr;  // device, per band (batched runs)
+  const double* band_t = nullptr;      // device, per band (batched runs)
+  double t_min = 0.0;                  // smallest per-band threshold
 };
 
 struct ScanOutput {
@@ -133,6 +137,8 @@ struct ScanOutput {
   double ms_scan = 0, ms_rescore = 0;
 };
 
+// Host-side argument checks (no device access).
+void check_csr_host(const apss_csr& X, int32_t p);
 // Upload (if needed) a CSR matrix and validate it; computes norms.
 Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p);
 // Histogram + tier selection + cursor lists of a corpus side.
@@ -140,7 +146,8 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 // Per-tile bitmap tier (+ cursor records when `records`) of a side against
 // corpus tiers, for tiles of at most max_rows rows and cap_d bitmap entries.
 TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
-                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records);
+                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records,
+                    const std::vector<int64_t>& bounds);
 // J tile of every cursor-list entry for the J tiling `tiles`.
 void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles);
 // Band scan + rescore; output pairs stay on the device.
@@ -150,6 +157,18 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
 
 // Full self-join into a result (host or device pairs per opts).
 int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
+// Many independent self-joins in one band-scan launch.
+int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
+                const apss_opts* o, apss_result* out);
+// Copy device pairs into a host result (optionally sorted).
+void finish_result(hipStream_t st, const apss_pair* dpairs, int64_t count, bool sort, apss_result* out);
+// Stream owned for the duration of a call when the caller gives none.
+struct StreamScope {
+  hipStream_t s = nullptr;
+  bool own = false;
+  StreamScope(void* given, int device);
+  ~StreamScope();
+};
 
 // Sorting and copying helpers.
 void sort_pairs(apss_pair* p, int64_t n);

@@ -132,7 +132,7 @@ static std::string verr_text(uint32_t e) {
 // ---------------------------------------------------------------------------
 // prepare_side: upload + validation + norms
 
-Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
+void check_csr_host(const apss_csr& X, int32_t p) {
   if (X.n_rows < 0 || X.n_cols < 0 || X.nnz < 0)
     throw Error{APSS_E_INVALID, "negative matrix dimension"};
   if (X.n_rows > 0 && (!X.row_ptr || (X.nnz > 0 && (!X.col || !X.val))))
@@ -140,6 +140,14 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
   if (X.n_rows >= (int64_t)0x7fffffff || X.n_cols >= (int64_t)0x7fffffff)
     throw Error{APSS_E_UNSUPPORTED, "more than 2^31-1 rows or columns"};
   if (p < 0 || p > X.n_cols) throw Error{APSS_E_INVALID, "p outside [0, n_cols]"};
+  if (!X.on_device && X.n_rows > 0) {
+    if (X.row_ptr[0] != 0) throw Error{APSS_E_INVALID, "row_ptr[0] != 0"};
+    if (X.row_ptr[X.n_rows] != X.nnz) throw Error{APSS_E_INVALID, "row_ptr[n_rows] != nnz"};
+  }
+}
+
+Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
+  check_csr_host(X, p);
   Side s;
   s.n = X.n_rows;
   s.nnz = X.nnz;
@@ -149,9 +157,6 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
     s.col = X.col;
     s.val = X.val;
   } else {
-    if (X.n_rows > 0 && X.row_ptr[0] != 0) throw Error{APSS_E_INVALID, "row_ptr[0] != 0"};
-    if (X.n_rows > 0 && X.row_ptr[X.n_rows] != X.nnz)
-      throw Error{APSS_E_INVALID, "row_ptr[n_rows] != nnz"};
     int64_t* rp = ar.alloc<int64_t>(s.n + 1);
     int32_t* col = ar.alloc<int32_t>(s.nnz);
     float* val = ar.alloc<float>(s.nnz);
@@ -300,14 +305,19 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 // entries (padded to a multiple of 4) stay within cap_d and whose cursor-tier
 // entries stay within cap_s (a single row always forms a tile).
 static std::vector<int64_t> cut_tiles(const std::vector<int32_t>& dc, const std::vector<int32_t>& sc,
-                                      int32_t max_rows, int32_t cap_d, int32_t cap_s) {
+                                      int32_t max_rows, int32_t cap_d, int32_t cap_s,
+                                      const std::vector<int64_t>& bounds) {
   std::vector<int64_t> ts;
   const int64_t n = (int64_t)dc.size();
   ts.push_back(0);
   int64_t rows = 0, sd = 0, ss = 0;
+  size_t nb = 0;  // next forced boundary (instance start)
+  while (nb < bounds.size() && bounds[nb] <= 0) ++nb;
   for (int64_t r = 0; r < n; ++r) {
     const int64_t nd = ((sd + dc[r] + 3) / 4) * 4;
-    if (rows > 0 && (rows == max_rows || nd > cap_d || ss + sc[r] > cap_s)) {
+    const bool forced = nb < bounds.size() && bounds[nb] == r;
+    if (forced) ++nb;
+    if (rows > 0 && (forced || rows == max_rows || nd > cap_d || ss + sc[r] > cap_s)) {
       ts.push_back(r);
       rows = sd = ss = 0;
     }
@@ -320,7 +330,8 @@ static std::vector<int64_t> cut_tiles(const std::vector<int32_t>& dc, const std:
 }
 
 TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
-                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records) {
+                    int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records,
+                    const std::vector<int64_t>& bounds) {
   TileSet ts;
   std::vector<int64_t> cuts(1, 0);
   if (s.n > 0) {
@@ -334,10 +345,11 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
     APSS_HIP(hipStreamSynchronize(st));
     ar.release(cnts);
     if (!records) std::fill(sc.begin(), sc.end(), 0);
-    cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s);
+    cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s, bounds);
   }
   const int32_t T = (int32_t)cuts.size() - 1;
   ts.T = T;
+  ts.cuts = cuts;
   ts.tstart = ar.alloc<int64_t>(T + 1);
   APSS_HIP(hipMemcpyAsync(ts.tstart, cuts.data(), sizeof(int64_t) * (T + 1), hipMemcpyHostToDevice, st));
   if (T >= 0xffff) throw Error{APSS_E_UNSUPPORTED, "more than 65534 tiles"};
@@ -437,7 +449,9 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   const double qscale = std::ldexp(1.0, S), qinv = std::ldexp(1.0, -S);
   const double head_eps = tr.p > 0 ? gamma32(maxlen + 1) * mI * mJ : 0.0;
   const double eps_max = cell_error_bound((int)maxlen, mI, mJ, qinv, head_eps);
-  const double qlo = std::floor((rc.t - 2.0 * eps_max) * qscale);
+  // fast-reject bound: the smallest threshold any band uses
+  const double t_min = rc.band_t ? rc.t_min : rc.t;
+  const double qlo = std::floor((t_min - 2.0 * eps_max) * qscale);
   const int32_t q_lo = (int32_t)std::max(-2147483647.0, std::min(2147483647.0, qlo));
 
   // pair-space size bounds the output capacity
@@ -480,6 +494,9 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.band_cnt = ctr + 1;
     a.band_cap = bcap;
     a.rescore_all = rc.rescore_all;
+    a.band_jend = rc.band_jend;
+    a.band_t = rc.band_t;
+    a.eps_max = eps_max;
     a.band_first = 0;
     a.band_count = bands.T;
     a.stats = ctr + 2;
@@ -553,6 +570,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   if (!X || !out) throw Error{APSS_E_INVALID, "null argument"};
   if (!(t == t) || std::isinf(t)) throw Error{APSS_E_INVALID, "t must be finite"};
   std::memset(out, 0, sizeof(*out));
+  check_csr_host(*X, p < 0 ? 0 : p);  // argument errors are reported before any device call
   APSS_HIP(hipSetDevice(o->device));
   hipStream_t st = (hipStream_t)o->stream;
   bool own_stream = false;
@@ -574,8 +592,8 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->ms_upload = tm.stop_ms();
   tm.start();
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true);
-  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false);
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true, {});
+  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, {});
   set_cursor_tiles(ar, st, tr, tiles);
   out->ms_index = tm.stop_ms();
   RunConfig rc;
@@ -621,6 +639,165 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   APSS_HIP(hipStreamSynchronize(st));
   out->ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return APSS_OK;
+}
+
+}  // namespace apss
+
+namespace apss {
+
+StreamScope::StreamScope(void* given, int device) {
+  APSS_HIP(hipSetDevice(device));
+  s = (hipStream_t)given;
+  if (!s) {
+    APSS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    own = true;
+  }
+}
+StreamScope::~StreamScope() {
+  if (own) (void)hipStreamDestroy(s);
+}
+
+void finish_result(hipStream_t st, const apss_pair* dpairs, int64_t count, bool sort,
+                   apss_result* out) {
+  apss_pair* h = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(count, 1));
+  if (!h) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+  if (count)
+    APSS_HIP(hipMemcpyAsync(h, dpairs, sizeof(apss_pair) * count, hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipStreamSynchronize(st));
+  if (sort) sort_pairs(h, count);
+  out->pairs = h;
+  out->count = count;
+  out->on_device = 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched: every instance's rows are concatenated with tile boundaries forced
+// at the instance starts; each band only meets the tiles of its own instance
+// (per-band partner range) and uses its instance's threshold, so one band-scan
+// launch covers the whole batch.
+
+int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
+                const apss_opts* o, apss_result* out) {
+  apss_opts def;
+  apss_opts_init(&def);
+  if (!o) o = &def;
+  if (count < 0 || (count > 0 && (!X || !t || !out))) throw Error{APSS_E_INVALID, "null argument"};
+  if (p < 0) p = 0;
+  int64_t n_all = 0, nnz_all = 0, d = 0;
+  double t_min = INFINITY;
+  for (int32_t k = 0; k < count; ++k) {
+    if (!X[k]) throw Error{APSS_E_INVALID, "null instance"};
+    if (X[k]->on_device) throw Error{APSS_E_UNSUPPORTED, "batched instances must be host CSR"};
+    check_csr_host(*X[k], p);
+    if (!(t[k] == t[k]) || std::isinf(t[k])) throw Error{APSS_E_INVALID, "t must be finite"};
+    n_all += X[k]->n_rows;
+    nnz_all += X[k]->nnz;
+    d = std::max(d, X[k]->n_cols);
+    t_min = std::min(t_min, t[k]);
+    std::memset(&out[k], 0, sizeof(apss_result));
+  }
+  if (count == 0) return APSS_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  // concatenation on the host
+  std::vector<int64_t> off((size_t)count + 1, 0), rp((size_t)n_all + 1, 0);
+  std::vector<int32_t> col((size_t)nnz_all);
+  std::vector<float> val((size_t)nnz_all);
+  int64_t r0 = 0, e0 = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    off[k] = r0;
+    const apss_csr& A = *X[k];
+    for (int64_t i = 0; i < A.n_rows; ++i) rp[r0 + i + 1] = e0 + A.row_ptr[i + 1];
+    if (A.nnz) {
+      std::memcpy(col.data() + e0, A.col, sizeof(int32_t) * A.nnz);
+      std::memcpy(val.data() + e0, A.val, sizeof(float) * A.nnz);
+    }
+    r0 += A.n_rows;
+    e0 += A.nnz;
+  }
+  off[count] = r0;
+  apss_csr all{n_all, d, nnz_all, rp.data(), col.data(), val.data(), 0, 0};
+  StreamScope ss(o->stream, o->device);
+  hipStream_t st = ss.s;
+  Arena ar(o->device, o->mem_budget);
+  Timer tm(st);
+  tm.start();
+  Side s = prepare_side(ar, st, all, p);
+  const double ms_upload = tm.stop_ms();
+  tm.start();
+  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
+  std::vector<int64_t> bounds(off.begin() + 1, off.end() - 1);
+  std::sort(bounds.begin(), bounds.end());
+  bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true, bounds);
+  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, bounds);
+  set_cursor_tiles(ar, st, tr, tiles);
+  // per band: instance, partner range end, threshold
+  std::vector<int32_t> hjend((size_t)std::max(bands.T, 1));
+  std::vector<double> ht((size_t)std::max(bands.T, 1));
+  for (int32_t b = 0; b < bands.T; ++b) {
+    const int64_t r = bands.cuts[b];
+    const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), r) - off.begin()) - 1;
+    const int64_t end_row = off[k + 1];
+    hjend[b] = (int32_t)(std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) - tiles.cuts.begin());
+    ht[b] = t[k];
+  }
+  int32_t* djend = ar.alloc<int32_t>(bands.T);
+  double* dt = ar.alloc<double>(bands.T);
+  if (bands.T > 0) {
+    APSS_HIP(hipMemcpyAsync(djend, hjend.data(), sizeof(int32_t) * bands.T, hipMemcpyHostToDevice, st));
+    APSS_HIP(hipMemcpyAsync(dt, ht.data(), sizeof(double) * bands.T, hipMemcpyHostToDevice, st));
+  }
+  const double ms_index = tm.stop_ms();
+  RunConfig rc;
+  rc.t = t_min;
+  rc.p = p;
+  rc.self = 1;
+  rc.rescore_all = o->rescore_all;
+  rc.band_jend = djend;
+  rc.band_t = dt;
+  rc.t_min = t_min;
+  ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
+  // split the pair list by instance
+  std::vector<apss_pair> h((size_t)so.count);
+  if (so.count)
+    APSS_HIP(hipMemcpyAsync(h.data(), so.pairs, sizeof(apss_pair) * so.count, hipMemcpyDeviceToHost, st));
+  APSS_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> cnt((size_t)count, 0);
+  std::vector<int32_t> inst((size_t)so.count);
+  for (int64_t q = 0; q < so.count; ++q) {
+    const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), (int64_t)h[q].i) - off.begin()) - 1;
+    inst[q] = k;
+    ++cnt[k];
+  }
+  for (int32_t k = 0; k < count; ++k) {
+    out[k].pairs = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(cnt[k], 1));
+    if (!out[k].pairs) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+    out[k].count = 0;
+  }
+  for (int64_t q = 0; q < so.count; ++q) {
+    apss_result& R = out[inst[q]];
+    apss_pair pr = h[q];
+    pr.i -= (uint32_t)off[inst[q]];
+    pr.j -= (uint32_t)off[inst[q]];
+    R.pairs[R.count++] = pr;
+  }
+  const double ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int32_t k = 0; k < count; ++k) {
+    if (o->sort_output) sort_pairs(out[k].pairs, out[k].count);
+    out[k].p_used = p;
+    out[k].cells = X[k]->n_rows * (X[k]->n_rows - 1) / 2;
+    out[k].band = so.band;
+    out[k].products = so.products;
+    out[k].list_visits = so.visits;
+    out[k].cursor_products = so.cursor;
+    out[k].ms_upload = ms_upload;
+    out[k].ms_index = ms_index;
+    out[k].ms_scan = so.ms_scan;
+    out[k].ms_rescore = so.ms_rescore;
+    out[k].ms_total = ms_total;
+  }
   return APSS_OK;
 }
 

@@ -120,6 +120,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   if (I >= a.I.T) return;
   const int64_t i0 = a.I.tstart[I];
   const int nI = (int)(a.I.tstart[I + 1] - i0);
+  // batched runs give each band its own partner range and threshold
+  const int32_t J_end = a.band_jend ? a.band_jend[I] : a.J.T;
+  const double t_band = a.band_t ? a.band_t[I] : a.t;
+  int32_t q_lo = a.q_lo;
+  if (a.band_t) {
+    const double ql = floor((t_band - 2.0 * a.eps_max) * (double)a.qscale);
+    q_lo = (int32_t)fmax(-2147483647.0, fmin(2147483647.0, ql));
+  }
 
   // ---- band set-up: this thread's band entries live in registers ---------
   // Entry k = tid + q * SCAN_THREADS of the band block: (row * TSTRIDE,
@@ -175,9 +183,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int32_t J_first = J_first_s;
 
   Prefetch pf;
-  if (J_first < a.J.T) prefetch_tile(a.J, J_first, KW, tid, pf);
+  if (J_first < J_end) prefetch_tile(a.J, J_first, KW, tid, pf);
 
-  for (int32_t J = J_first; J < a.J.T; ++J) {
+  for (int32_t J = J_first; J < J_end; ++J) {
     // per-thread addresses are recomputed from an opaque thread index in every
     // pass instead of being kept live across the loop (register budget)
     int tl = tid;
@@ -215,7 +223,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     __syncthreads();
 
     // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
-    if (J + 1 < a.J.T) prefetch_tile(a.J, J + 1, KW, tl, pf);
+    if (J + 1 < J_end) prefetch_tile(a.J, J + 1, KW, tl, pf);
 
     // ---- 1. cursor tier, first half: find this thread's first cursor in J
     // and issue its loads; they complete behind the bitmap-tier products ----
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
         const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jl];
         if (bound + cell_error_bound(lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl], normI[il], normJ[jl],
-                                     a.qinv, 0.0) < a.t)
+                                     a.qinv, 0.0) < t_band)
           continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
@@ -337,7 +345,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       const int32_t q = tile[il * TSTRIDE + jl];
       const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
       my_cells += valid ? 1u : 0u;
-      cand |= (valid && q >= a.q_lo) ? (1u << m) : 0u;
+      cand |= (valid && q >= q_lo) ? (1u << m) : 0u;
     }
 #pragma unroll 1
     for (int m = 0; m < CELLS_PER_THREAD; ++m) {
@@ -355,10 +363,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jl] : 0.0;
         const double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
         sd = (double)q * a.qinv;
-        if (sd - eps >= a.t) {
+        if (sd - eps >= t_band) {
           // certain hit; its score is final only if the bound meets the relative tolerance
           if (a.rescore_all || eps > a.score_rel * (sd - eps)) to_band = true; else emit = true;
-        } else if (sd + eps >= a.t) {
+        } else if (sd + eps >= t_band) {
           to_band = true;
         }
       }
@@ -394,7 +402,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     __syncthreads();
     // flush the LDS output buffer when half full, and after the last tile
     const unsigned int nbuf = obuf_n;
-    if (nbuf >= OBUF_CAP / 2 || (J + 1 == a.J.T && nbuf > 0)) {
+    if (nbuf >= OBUF_CAP / 2 || (J + 1 == J_end && nbuf > 0)) {
       const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
       if (tl == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
       __syncthreads();

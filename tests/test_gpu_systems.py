@@ -1,0 +1,146 @@
+"""Batched runs, the corpus index (queries and inserts), the continuous-batching
+server and the reference-signature shim, each against the FP64 oracle."""
+import threading
+
+import numpy as np
+import pytest
+
+import apss
+import oracle
+from helpers import check_pairs, subset
+
+pytestmark = pytest.mark.gpu
+
+
+def small_rows(n, seed_offset=0, name="small"):
+    rp, col, val, d = apss.workload(name, n=n, seed_offset=seed_offset)
+    return rp, col, val, d
+
+
+def test_batched_each_instance_matches_alone():
+    sizes = [700, 0, 1, 2300, 64, 1500]
+    ts = [0.5, 0.3, 0.9, 0.45, 0.2, 0.6]
+    inst = [small_rows(n, seed_offset=10 + k) for k, n in enumerate(sizes)]
+    res = apss.run_batched(inst, ts, p=0)
+    for (rp, col, val, d), t, (pairs, info) in zip(inst, ts, res):
+        exp = oracle.apss_fp64(rp, col, val, d, t)
+        check_pairs(pairs, exp)
+
+
+def test_batched_with_split_and_small_bitmap_tier():
+    inst = [small_rows(n, seed_offset=30 + k, name="large") for k, n in enumerate([900, 1300, 400])]
+    ts = [0.4, 0.5, 0.35]
+    res = apss.run_batched(inst, ts, p=50, dense_features=64)
+    for (rp, col, val, d), t, (pairs, _) in zip(inst, ts, res):
+        check_pairs(pairs, oracle.apss_fp64(rp, col, val, d, t))
+
+
+@pytest.mark.parametrize("p", [0, 64])
+def test_index_query_matches_oracle(p):
+    crp, ccol, cval, d = small_rows(3000, name="serving")
+    qrp, qcol, qval, _ = small_rows(400, seed_offset=100, name="serving")
+    ix = apss.Index(crp, ccol, cval, d, p=p)
+    try:
+        pairs, info = ix.query(qrp, qcol, qval, 0.5)
+    finally:
+        ix.close()
+    exp = oracle.query_fp64((qrp, qcol, qval), (crp, ccol, cval), d, 0.5)
+    check_pairs(pairs, exp)
+
+
+def test_index_inserts_and_merge():
+    crp, ccol, cval, d = small_rows(2000, name="serving")
+    ix = apss.Index(crp, ccol, cval, d)
+    allr = [(crp, ccol, cval)]
+    try:
+        for k in range(6):  # more inserts than segments kept: forces a merge
+            rp, col, val, _ = small_rows(150 + 37 * k, seed_offset=200 + k, name="serving")
+            ix.insert(rp, col, val)
+            allr.append((rp, col, val))
+        n_all = sum(len(r[0]) - 1 for r in allr)
+        assert ix.rows() == n_all
+        qrp, qcol, qval, _ = small_rows(300, seed_offset=300, name="serving")
+        pairs, _ = ix.query(qrp, qcol, qval, 0.45)
+    finally:
+        ix.close()
+    lens = np.concatenate([np.diff(r[0]) for r in allr])
+    crp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ccol2 = np.concatenate([r[1] for r in allr])
+    cval2 = np.concatenate([r[2] for r in allr])
+    exp = oracle.query_fp64((qrp, qcol, qval), (crp2, ccol2, cval2), d, 0.45)
+    check_pairs(pairs, exp)
+
+
+def test_server_result_same_alone_and_in_the_mix():
+    crp, ccol, cval, d = small_rows(2500, name="serving")
+    ix = apss.Index(crp, ccol, cval, d)
+    sizes = [1, 37, 600, 5, 250, 1200, 64, 3]
+    ts = [0.6, 0.8, 0.6, 0.5, 0.8, 0.6, 0.7, 0.9]
+    qs = [small_rows(n, seed_offset=400 + k, name="serving")[:3] for k, n in enumerate(sizes)]
+    try:
+        alone = [ix.query(*q, t)[0] for q, t in zip(qs, ts)]
+        srv = apss.Server(ix, slots=2)
+        try:
+            tickets = [None] * len(qs)
+
+            def submit(k):
+                tickets[k] = srv.submit(*qs[k], ts[k])
+
+            th = [threading.Thread(target=submit, args=(k,)) for k in range(len(qs))]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
+            mixed = [srv.wait(tk) for tk in tickets]
+        finally:
+            srv.close()
+    finally:
+        ix.close()
+    for q, t, a, (m, lat) in zip(qs, ts, alone, mixed):
+        exp = oracle.query_fp64(q, (crp, ccol, cval), d, t)
+        check_pairs(a, exp)
+        check_pairs(m, exp)
+        assert lat >= 0
+
+
+def test_server_cancel_and_budget():
+    crp, ccol, cval, d = small_rows(800, name="serving")
+    ix = apss.Index(crp, ccol, cval, d)
+    try:
+        srv = apss.Server(ix, slots=1, mem_budget=1 << 20)  # 1 MiB: every request is too large
+        try:
+            q = small_rows(10, seed_offset=500, name="serving")[:3]
+            with pytest.raises(apss.ApssError) as e:
+                srv.submit(*q, 0.5)
+            assert e.value.code == apss.APSS_E_NOMEM
+        finally:
+            srv.close()
+        srv = apss.Server(ix, slots=1, max_queue=64)
+        try:
+            tks = [srv.submit(*small_rows(300, seed_offset=600 + k, name="serving")[:3], 0.5) for k in range(6)]
+            cancelled = [tk for tk in tks[3:] if srv.cancel(tk) == apss.APSS_OK]
+            for tk in tks:
+                if tk in cancelled:
+                    with pytest.raises(apss.ApssError) as e:
+                        srv.wait(tk)
+                    assert e.value.code == apss.APSS_E_CANCELLED
+                else:
+                    srv.wait(tk)
+        finally:
+            srv.close()
+    finally:
+        ix.close()
+
+
+def test_reference_signature_shim_counts_pairs():
+    rp, col, val, d = small_rows(2400)
+    # eight partitions, the last one smaller (as in main.cu:180-187)
+    cuts = [0, 300, 600, 900, 1200, 1500, 1800, 2100, 2400]
+    parts = [subset(*[x for x in (rp, col, val)], 0)] * 0
+    parts = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        prp = (rp[a:b + 1] - rp[a]).astype(np.int64)
+        parts.append((prp, col[rp[a]:rp[b]].copy(), val[rp[a]:rp[b]].copy()))
+    count = apss.similarity_join_device(parts)
+    exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
+    assert count == len(exp)
