@@ -105,7 +105,7 @@ struct ScanArgs {
   int32_t self;              // 1: J >= I and only i < j
   int32_t head_mode;         // 0: no head (p == 0), 1: forward-list dot
   double t;
-  // Score tile cells are fixed point: a product x*y adds rint(x * 2^S * y)
+  // Score tile cells are fixed point: a product x*y adds trunc(x * 2^S * y)
   // (int32).  Partial sums stay below 2^31 because 2^S * |x||y| <= 2^30.
   float qscale;              // 2^S
   int32_t q_lo;              // cells below this cannot reach t (fast reject)
@@ -187,12 +187,13 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long* counter, b
 }
 
 // Rigorous bound of |s_tile - s| for one cell: FP32 product rounding (u per
-// product, summed <= u*|x||y|), fixed-point rounding (half a quantum per
-// product and one for the head), and the FP32 head dot when there is one.
+// product, summed <= u*|x||y|), fixed-point truncation (under one quantum
+// 2^-S per product and one for the head), and the FP32 head dot when there is
+// one.
 __device__ __host__ __forceinline__ double cell_error_bound(int k, double norm_i, double norm_j,
                                                             double qinv, double head_bound) {
   const double u = 5.9604644775390625e-08;
-  return u * norm_i * norm_j * (1.0 + 1e-6) + (double)(k + 2) * 0.5 * qinv + head_bound;
+  return u * norm_i * norm_j * (1.0 + 1e-6) + (double)(k + 2) * qinv + head_bound;
 }
 
 // Unit roundoff bound gamma_k = k*u/(1-k*u) for FP32 (u = 2^-24).

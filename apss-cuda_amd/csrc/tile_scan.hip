@@ -172,11 +172,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   }
   const int64_t sb = a.I.sbase[I];
   const int nS = (int)(a.I.sbase[I + 1] - sb);
+  int32_t my_due = 0xffff;  // first tile any of this thread's cursors points into
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     int4 r = a.I.srec[sb + k];
     int32_t nt = r.z < r.w ? (int32_t)a.S.tile[r.z] : 0xffff;
     if (k < CAP_SI) { s_nt[k] = (uint16_t)nt; s_cur[k] = r.z; }
     else { a.snext_spill[sb + k] = nt; a.scur[sb + k] = r.z; }
+    my_due = nt < my_due ? nt : my_due;
   }
   unsigned int my_products = 0, my_head = 0, my_cells = 0, my_visits = 0, my_cursor = 0;
   __syncthreads();
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     int hk = -1;
     int4 hrec = make_int4(0, 0, 0, 0);
     int2 hent = make_int2(0, 0);
-    for (int k = tl; k < nS; k += SCAN_THREADS) {
+    for (int k = my_due == J ? tl : nS; k < nS; k += SCAN_THREADS) {
       const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
       if (nt == J) {
         hk = k;
@@ -269,9 +271,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         my_visits += 1u + (((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u);
         const int ib = (e_rf[h + q] >> 12) & 0x3fff;
         const float u = vI[tl + (h + q) * SCAN_THREADS];
-        for (int y = y0[q]; y < y1[q]; ++y) {
+        // four entries per step: their LDS reads issue together, then the atomics
+        int y = y0[q];
+        for (; y + 4 <= y1[q]; y += 4) {
+          const int2 e0 = eJ[y], e1 = eJ[y + 1], e2 = eJ[y + 2], e3 = eJ[y + 3];
+          atomicAdd(&tile[ib + e0.x], (int)(u * __int_as_float(e0.y)));
+          atomicAdd(&tile[ib + e1.x], (int)(u * __int_as_float(e1.y)));
+          atomicAdd(&tile[ib + e2.x], (int)(u * __int_as_float(e2.y)));
+          atomicAdd(&tile[ib + e3.x], (int)(u * __int_as_float(e3.y)));
+        }
+        for (; y < y1[q]; ++y) {
           const int2 e = eJ[y];
-          atomicAdd(&tile[ib + e.x], __float2int_rn(u * __int_as_float(e.y)));
+          atomicAdd(&tile[ib + e.x], (int)(u * __int_as_float(e.y)));
         }
         my_products += (unsigned)(y1[q] - y0[q]);
       }
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       const int rowbase = r.x * TSTRIDE - (int)j0;
       int32_t ntn = 0xffff;
       for (;;) {
-        atomicAdd(&tile[rowbase + e.x], __float2int_rn(x * __int_as_float(e.y)));
+        atomicAdd(&tile[rowbase + e.x], (int)(x * __int_as_float(e.y)));
         ++my_products;
         ++my_cursor;
         if (++cur >= r.w) break;
@@ -307,6 +318,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       }
       if (k < CAP_SI) { s_nt[k] = (uint16_t)ntn; s_cur[k] = cur; }
       else { a.snext_spill[sb + k] = ntn; a.scur[sb + k] = cur; }
+    }
+    if (hk >= 0) {  // some cursors moved: recompute this thread's next due tile
+      my_due = 0xffff;
+      for (int k = tl; k < nS; k += SCAN_THREADS) {
+        const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
+        my_due = nt < my_due ? nt : my_due;
+      }
     }
     __syncthreads();
 
@@ -336,14 +354,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // ---- 4. threshold, compaction, reset -------------------------------------
     // Pass 1 reads this thread's cells together and marks the ones that can
     // reach t; pass 2 (rolled) settles candidates and clears every cell.
+    // cell c = tl + m*SCAN_THREADS: column tl % TILE_ROWS is fixed per thread,
+    // row advances by SCAN_THREADS / TILE_ROWS; a cell is valid when its row
+    // is below both the band height and (self-joins) the diagonal limit
     constexpr int CELLS_PER_THREAD = BAND_ROWS * TILE_ROWS / SCAN_THREADS;
+    constexpr int ROW_STEP = SCAN_THREADS / TILE_ROWS;
+    static_assert(SCAN_THREADS % TILE_ROWS == 0, "fixed column per thread");
+    const int jl_c = tl % TILE_ROWS, il_0 = tl / TILE_ROWS;
+    int row_lim = jl_c < nJ ? nI : 0;
+    if (a.self) {  // i0 + il < j0 + jl  <=>  il < (j0 - i0) + jl
+      const int64_t dlim = j0 - i0 + jl_c;
+      row_lim = (int)(dlim < row_lim ? (dlim < 0 ? 0 : dlim) : row_lim);
+    }
     unsigned cand = 0;
 #pragma unroll
     for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-      const int c = tl + m * SCAN_THREADS;
-      const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
-      const int32_t q = tile[il * TSTRIDE + jl];
-      const bool valid = il < nI && jl < nJ && !(a.self && i0 + il >= j0 + jl);
+      const int il = il_0 + m * ROW_STEP;
+      const int32_t q = tile[il * TSTRIDE + jl_c];
+      const bool valid = il < row_lim;
       my_cells += valid ? 1u : 0u;
       cand |= (valid && q >= q_lo) ? (1u << m) : 0u;
     }
