@@ -29,8 +29,9 @@ constexpr int KD_MAX = 4096;                 // features in the bitmap tier
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
 constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
-constexpr int CAP_SI = 6144;                 // cursor-tier entries of a band kept in LDS
+constexpr int CAP_S = 8192;                  // cursor-tier records of a band (band cut)
 constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
+constexpr int CH_TILES = 1024;               // J tiles per cursor-bucket chunk
 
 // Feature classes in fmap[] (values >= 0 are bitmap-tier indices).
 constexpr int32_t F_HEAD = -1;    // id < p: head (forward list / MFMA)
@@ -95,7 +96,7 @@ struct TileSide {
 struct CursorLists {
   const int64_t* ptr = nullptr;   // d+1
   const int2* ent = nullptr;
-  const uint16_t* tile = nullptr; // J tile of each entry's row
+  const uint32_t* pos = nullptr;  // J tile << 8 | row within the tile, per entry
 };
 
 struct ScanArgs {
@@ -128,7 +129,12 @@ struct ScanArgs {
   const double* band_t;      // per band: threshold (NULL: t)
   double eps_max;            // largest cell error bound (for per-band fast rejects)
   int32_t* scur;             // cursor position of every cursor record (I side)
-  int32_t* snext_spill;      // next tile of records beyond CAP_SI
+  int32_t* snext;            // J tile the cursor points into (INT32_MAX: exhausted)
+  // cursor-tier products, bucketed by J tile for one chunk of CH_TILES tiles;
+  // one region of cprod_cap records per workgroup of the persistent grid
+  int2* cprod;               // {cell = il * TSTRIDE + jl, fixed-point product}
+  int64_t cprod_cap;
+  unsigned long long* band_next;  // band queue of the persistent grid
 };
 
 // Arguments of the per-tile bitmap-tier build kernels.

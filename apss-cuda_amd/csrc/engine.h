@@ -26,7 +26,8 @@ __global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
 __global__ void k_tile_compact(BuildSide s);
 __global__ void k_row2tile(const int64_t* tstart, int32_t* row2tile);
-__global__ void k_cursor_tiles(int64_t n, const int2* sent, const int32_t* row2tile, uint16_t* stile);
+__global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
+                             const int64_t* tstart, uint32_t* spos);
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
@@ -73,7 +74,7 @@ struct Tiers {
   int32_t* fmap = nullptr;       // d
   int64_t* sptr = nullptr;       // d+1
   int2* sent = nullptr;          // {row, value bits}
-  uint16_t* stile = nullptr;     // J tile of each entry's row (set per J tiling)
+  uint32_t* spos = nullptr;      // J tile << 8 | row in tile, per entry (set per J tiling)
 };
 
 // One matrix side: CSR on the device and its norms.
@@ -104,6 +105,7 @@ struct TileSet {
   int64_t* sbase = nullptr;
   int4* srec = nullptr;
   int64_t nD = 0, nS = 0;
+  int64_t max_s = 0;             // most cursor records in one tile
   std::vector<int64_t> cuts;     // host copy of tstart
 };
 
@@ -148,7 +150,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,
                     int32_t max_rows, int32_t cap_d, int32_t cap_s, bool records,
                     const std::vector<int64_t>& bounds);
-// J tile of every cursor-list entry for the J tiling `tiles`.
+// J tile and row in the tile of every cursor-list entry for the J tiling `tiles`.
 void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles);
 // Band scan + rescore; output pairs stay on the device.
 ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,

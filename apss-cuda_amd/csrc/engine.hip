@@ -346,6 +346,11 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
     ar.release(cnts);
     if (!records) std::fill(sc.begin(), sc.end(), 0);
     cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s, bounds);
+    for (size_t k = 0; k + 1 < cuts.size(); ++k) {
+      int64_t m = 0;
+      for (int64_t r = cuts[k]; r < cuts[k + 1]; ++r) m += sc[r];
+      ts.max_s = std::max(ts.max_s, m);
+    }
   }
   const int32_t T = (int32_t)cuts.size() - 1;
   ts.T = T;
@@ -425,11 +430,11 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
 }
 
 void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles) {
-  if (tr.stile) ar.release(tr.stile);
-  tr.stile = ar.alloc<uint16_t>(tr.n_cursor_entries);
+  if (tr.spos) ar.release(tr.spos);
+  tr.spos = ar.alloc<uint32_t>(tr.n_cursor_entries);
   if (tr.n_cursor_entries > 0) {
-    hipLaunchKernelGGL(k_cursor_tiles, dim3(blocks_for(tr.n_cursor_entries, 256, 65536)), dim3(256),
-                       0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tr.stile);
+    hipLaunchKernelGGL(k_cursor_pos, dim3(blocks_for(tr.n_cursor_entries, 256, 65536)), dim3(256),
+                       0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tiles.tstart, tr.spos);
     APSS_HIP(hipGetLastError());
   }
 }
@@ -463,7 +468,17 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   int64_t bcap = std::max<int64_t>(1 << 20, cap / 64);
   unsigned long long* ctr = ar.alloc<unsigned long long>(8);
   int32_t* scur = ar.alloc<int32_t>(bands.nS);
-  int32_t* spill = ar.alloc<int32_t>(bands.nS);
+  int32_t* snext = ar.alloc<int32_t>(bands.nS);
+  // persistent grid: one 1024-thread workgroup per CU pulls bands from a queue
+  // and owns one cursor-product region, large enough for every product one
+  // tile pair can receive (records x tile rows)
+  int n_cu = 0, dev = 0;
+  APSS_HIP(hipGetDevice(&dev));
+  APSS_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::max(1, std::min(bands.T, std::max(n_cu, 1)));
+  const int64_t cprod_cap =
+      bands.nS > 0 ? std::max<int64_t>((int64_t)1 << 19, bands.max_s * TILE_ROWS) : 0;
+  int2* cprod = ar.alloc<int2>(cprod_cap * grid);
   Timer tm(st);
   for (int attempt = 0; attempt < 3; ++attempt) {
     apss_pair* out = ar.alloc<apss_pair>(cap);
@@ -474,7 +489,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.J = make_view(J, tiles);
     a.S.ptr = tr.sptr;
     a.S.ent = tr.sent;
-    a.S.tile = tr.stile;
+    a.S.pos = tr.spos;
     a.KD = tr.KD;
     a.KW = tr.KW;
     a.p = tr.p;
@@ -501,10 +516,13 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.band_count = bands.T;
     a.stats = ctr + 2;
     a.scur = scur;
-    a.snext_spill = spill;
+    a.snext = snext;
+    a.cprod = cprod;
+    a.cprod_cap = cprod_cap;
+    a.band_next = ctr + 7;
     tm.start();
     if (bands.T > 0 && tiles.T > 0) {
-      hipLaunchKernelGGL(k_band_scan, dim3(bands.T), dim3(SCAN_THREADS), 0, st, a);
+      hipLaunchKernelGGL(k_band_scan, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
       APSS_HIP(hipGetLastError());
     }
     so.ms_scan = tm.stop_ms();
@@ -592,7 +610,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->ms_upload = tm.stop_ms();
   tm.start();
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true, {});
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_S, true, {});
   TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, {});
   set_cursor_tiles(ar, st, tr, tiles);
   out->ms_index = tm.stop_ms();
@@ -729,7 +747,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   std::vector<int64_t> bounds(off.begin() + 1, off.end() - 1);
   std::sort(bounds.begin(), bounds.end());
   bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, INT32_MAX, true, bounds);
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_S, true, bounds);
   TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, bounds);
   set_cursor_tiles(ar, st, tr, tiles);
   // per band: instance, partner range end, threshold

@@ -385,13 +385,18 @@ __global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {
   if (threadIdx.x == 0) crow[wp[s.KW]] = prow[s.KD];
 }
 
-// J tile of the row of every cursor-list entry.
-__global__ __launch_bounds__(256) void k_cursor_tiles(int64_t n, const int2* __restrict__ sent,
-                                                      const int32_t* __restrict__ row2tile,
-                                                      uint16_t* __restrict__ stile) {
+// J tile and row within that tile of every cursor-list entry, packed as
+// tile << 8 | local row (tiles have at most 64 rows, tile ids fit 24 bits).
+__global__ __launch_bounds__(256) void k_cursor_pos(int64_t n, const int2* __restrict__ sent,
+                                                    const int32_t* __restrict__ row2tile,
+                                                    const int64_t* __restrict__ tstart,
+                                                    uint32_t* __restrict__ spos) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x)
-    stile[e] = (uint16_t)row2tile[sent[e].x];
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = sent[e].x;
+    const int32_t t = row2tile[r];
+    spos[e] = ((uint32_t)t << 8) | (uint32_t)(r - tstart[t]);
+  }
 }
 
 // Tile index of every row.

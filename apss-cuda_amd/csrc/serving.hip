@@ -121,7 +121,7 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
   for (double x : t_req) t_min = std::min(t_min, x);
   for (const auto& sg : ix.segs) {
     tm.start();
-    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, CAP_I, INT32_MAX, true, bounds);
+    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, CAP_I, CAP_S, true, bounds);
     std::vector<double> ht((size_t)std::max(bands.T, 1), t_min);
     for (int32_t b = 0; b < bands.T; ++b) {
       const int64_t r = bands.cuts[b];

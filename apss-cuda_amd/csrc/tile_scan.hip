@@ -95,29 +95,125 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int 
   }
 }
 
+// Exclusive scan over the workgroup (one value per thread); *total gets the
+// sum.  wsum holds 17 words of LDS.
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, int tl, uint32_t* wsum,
+                                                 uint32_t* total) {
+  const int lane = lane_id(), w = tl >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (tl == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < SCAN_THREADS / 64; ++k) {
+      const uint32_t c = wsum[k];
+      wsum[k] = acc;
+      acc += c;
+    }
+    wsum[SCAN_THREADS / 64] = acc;
+  }
+  __syncthreads();
+  *total = wsum[SCAN_THREADS / 64];
+  return wsum[w] + x - v;
+}
+
+// Cursor walks over one record's list from position cur, visiting the entries
+// whose J tile lies below ce (lists are row-sorted, so those come first).
+// Positions are read four at a time so one memory round trip covers up to
+// four entries.
+__device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
+                                           int32_t cs, int32_t ce, uint32_t* bkt) {
+  for (;;) {
+    uint32_t pt[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pt[q] = cur + q < end ? pos[cur + q] : 0xffffffffu;
+    bool go = true;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t t = (int32_t)(pt[q] >> 8);
+      go = go && t < ce;
+      if (go) atomicAdd(&bkt[t - cs], 1u);
+    }
+    if (!go) return;
+    cur += 4;
+  }
+}
+
+__device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
+                                            const int2* __restrict__ ent, int cur, int end,
+                                            int32_t cs, int32_t ce, int rowc, float x,
+                                            uint32_t* bfill, int2* __restrict__ cprod,
+                                            unsigned int& n_prod) {
+  for (;;) {
+    uint32_t pt[4];
+    int2 e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool ok = cur + q < end;
+      pt[q] = ok ? pos[cur + q] : 0xffffffffu;
+      e[q] = ok ? ent[cur + q] : make_int2(0, 0);
+    }
+    bool go = true;
+    int used = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t t = (int32_t)(pt[q] >> 8);
+      go = go && t < ce;
+      if (go) {
+        const uint32_t slot = atomicAdd(&bfill[t - cs], 1u);
+        cprod[slot] = make_int2(rowc + (int)(pt[q] & 0xffu), (int)(x * __int_as_float(e[q].y)));
+        ++used;
+      }
+    }
+    cur += used;
+    n_prod += (unsigned)used;
+    if (!go) return cur;
+  }
+}
+
+// Persistent band kernel: one workgroup per CU takes bands from a queue.
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ int32_t tile[BAND_ROWS * TSTRIDE];     // fixed-point scores (x 2^S)
   __shared__ uint64_t bJ[KW_MAX];
   __shared__ uint16_t wpJ[KW_MAX];
   __shared__ uint16_t cJ[KD_MAX + 1];
   __shared__ __align__(16) int2 eJ[CAP_J];          // {local row, value bits}
-  __shared__ uint16_t s_nt[CAP_SI];                  // next J tile of each cursor
-  __shared__ int32_t s_cur[CAP_SI];                  // its position in the cursor list
+  __shared__ uint32_t bkt[CH_TILES + 1];             // cursor-product bucket starts of the chunk
+  __shared__ uint32_t bfill[CH_TILES];               // bucket fill pointers while scattering
+  __shared__ uint32_t wsum[SCAN_THREADS / 64 + 1];
   __shared__ apss_pair obuf[OBUF_CAP];               // emitted pairs waiting for a flush
   __shared__ unsigned int obuf_n;
   __shared__ unsigned long long obuf_base;
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
   __shared__ float vI[CAP_I];                        // band entry values * 2^S, [q][tid]
-  __shared__ int32_t J_first_s;
+  __shared__ int32_t band_s, J_first_s, ch_end_s;
   __shared__ unsigned long long acc_products, acc_head, acc_cells, acc_visits, acc_cursor;
+  static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
 
-  const int tid = threadIdx.x;
-  const int lane = lane_id();
+  const int tid0 = threadIdx.x;
   const int KW = a.KW;
+  int2* const cprod = a.cprod + (int64_t)blockIdx.x * a.cprod_cap;
 
-  const int32_t I = a.band_first + blockIdx.x;
-  if (I >= a.I.T) return;
+  // the tile is cleared cell by cell by the threshold pass, so once is enough
+  for (int c = tid0; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
+  if (tid0 == 0) {
+    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
+  }
+  unsigned int my_products = 0, my_head = 0, my_cells = 0, my_visits = 0, my_cursor = 0;
+
+  for (;;) {
+  int tid = tid0;
+  asm volatile("" : "+v"(tid));
+  if (tid == 0) band_s = a.band_first + (int32_t)atomicAdd(a.band_next, 1ull);
+  __syncthreads();
+  const int32_t I = band_s;
+  if (I >= a.band_first + a.band_count || I >= a.I.T) break;
   const int64_t i0 = a.I.tstart[I];
   const int nI = (int)(a.I.tstart[I + 1] - i0);
   // batched runs give each band its own partner range and threshold
@@ -149,7 +245,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       vI[k] = ok ? a.I.dval[d0 + k] * a.qscale : 0.f;  // exact: power of 2
     }
   }
-  for (int c = tid; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid < BAND_ROWS) {
     bool ok = tid < nI;
     int64_t r = i0 + tid;
@@ -158,7 +253,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
   }
   if (tid == 0) {
-    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
     int32_t jf = 0;
     if (a.self) {  // first tile whose rows end after the band's first row
       int32_t lo = 0, hi = a.J.T;
@@ -170,31 +264,67 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
     J_first_s = jf;
   }
+  // cursor records: position and the J tile it points into
   const int64_t sb = a.I.sbase[I];
   const int nS = (int)(a.I.sbase[I + 1] - sb);
-  int32_t my_due = 0xffff;  // first tile any of this thread's cursors points into
   for (int k = tid; k < nS; k += SCAN_THREADS) {
-    int4 r = a.I.srec[sb + k];
-    int32_t nt = r.z < r.w ? (int32_t)a.S.tile[r.z] : 0xffff;
-    if (k < CAP_SI) { s_nt[k] = (uint16_t)nt; s_cur[k] = r.z; }
-    else { a.snext_spill[sb + k] = nt; a.scur[sb + k] = r.z; }
-    my_due = nt < my_due ? nt : my_due;
+    const int4 r = a.I.srec[sb + k];
+    a.scur[sb + k] = r.z;
+    a.snext[sb + k] = r.z < r.w ? (int32_t)(a.S.pos[r.z] >> 8) : INT32_MAX;
   }
-  unsigned int my_products = 0, my_head = 0, my_cells = 0, my_visits = 0, my_cursor = 0;
   __syncthreads();
   const int32_t J_first = J_first_s;
 
   Prefetch pf;
   if (J_first < J_end) prefetch_tile(a.J, J_first, KW, tid, pf);
+  int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
 
   for (int32_t J = J_first; J < J_end; ++J) {
     // per-thread addresses are recomputed from an opaque thread index in every
     // pass instead of being kept live across the loop (register budget)
     int tl = tid;
     asm volatile("" : "+v"(tl));
+
+    // ---- 1. cursor tier: bucket the products of the next chunk of tiles ----
+    // Every record walks its row-sorted list up to the chunk end twice: once
+    // to count the products per J tile, once to write them (fixed point, with
+    // their cell) into this workgroup's region, grouped by tile.  A chunk
+    // whose products exceed the region ends early at a tile boundary.
+    if (J == ch_hi) {
+      ch_lo = J;
+      const int32_t ce0 = J + CH_TILES < J_end ? J + CH_TILES : J_end;
+      bkt[tl] = 0;
+      if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; }
+      __syncthreads();
+      for (int k = tl; k < nS; k += SCAN_THREADS) {
+        if (a.snext[sb + k] >= ce0) continue;
+        walk_count(a.S.pos, a.scur[sb + k], a.I.srec[sb + k].w, ch_lo, ce0, bkt);
+      }
+      __syncthreads();
+      const uint32_t cnt = bkt[tl];
+      uint32_t total;
+      const uint32_t ex = block_exscan(cnt, tl, wsum, &total);
+      bkt[tl] = ex;
+      bfill[tl] = ex;
+      if (tl == 0) bkt[CH_TILES] = total;
+      if ((int64_t)ex + cnt > a.cprod_cap) atomicMin(&ch_end_s, ch_lo + tl);
+      __syncthreads();
+      const int32_t ce = ch_end_s;
+      for (int k = tl; k < nS; k += SCAN_THREADS) {
+        if (a.snext[sb + k] >= ce) continue;
+        const int4 r = a.I.srec[sb + k];
+        const int cur = walk_scatter(a.S.pos, a.S.ent, a.scur[sb + k], r.w, ch_lo, ce,
+                                     r.x * TSTRIDE, __int_as_float(r.y) * a.qscale, bfill, cprod,
+                                     my_cursor);
+        a.scur[sb + k] = cur;
+        a.snext[sb + k] = cur < r.w ? (int32_t)(a.S.pos[cur] >> 8) : INT32_MAX;
+      }
+      __syncthreads();
+      ch_hi = ce;
+    }
+
     const int64_t j0 = a.J.tstart[J];
     const int nJ = pf.nrows;
-    const int64_t j1 = j0 + nJ;
 
     // ---- write the prefetched tile J into LDS --------------------------------
     if (tl < KW) { bJ[tl] = pf.bmp; wpJ[tl] = pf.wpre; }
@@ -227,21 +357,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
     if (J + 1 < J_end) prefetch_tile(a.J, J + 1, KW, tl, pf);
 
-    // ---- 1. cursor tier, first half: find this thread's first cursor in J
-    // and issue its loads; they complete behind the bitmap-tier products ----
-    int hk = -1;
-    int4 hrec = make_int4(0, 0, 0, 0);
-    int2 hent = make_int2(0, 0);
-    for (int k = my_due == J ? tl : nS; k < nS; k += SCAN_THREADS) {
-      const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
-      if (nt == J) {
-        hk = k;
-        const int32_t cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
-        hrec = a.I.srec[sb + k];
-        hent = a.S.ent[cur];
-        break;
-      }
-    }
+    // this thread's first cursor product of tile J: its load completes behind
+    // the bitmap-tier products
+    int32_t pb = 0, pe = 0;
+    if (nS > 0) { pb = (int32_t)bkt[J - ch_lo]; pe = (int32_t)bkt[J - ch_lo + 1]; }
+    const int32_t k0 = pb + tl;
+    int2 hp = make_int2(0, 0);
+    if (k0 < pe) hp = cprod[k0];
 
     // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
     // Two passes of I_PER_THREAD/2 entries: the LDS lookups of a pass are
@@ -288,46 +410,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       }
     }
 
-    // ---- 1. cursor tier, second half: walk the cursors that point into J ---
-    for (int k = hk; k >= 0 && k < nS; k += SCAN_THREADS) {
-      int32_t cur;
-      int4 r;
-      int2 e;
-      if (k == hk) {  // loads issued before the bitmap phase
-        cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
-        r = hrec;
-        e = hent;
-      } else {
-        const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
-        if (nt != J) continue;
-        cur = k < CAP_SI ? s_cur[k] : a.scur[sb + k];
-        r = a.I.srec[sb + k];
-        e = a.S.ent[cur];
-      }
-      const float x = __int_as_float(r.y) * a.qscale;
-      const int rowbase = r.x * TSTRIDE - (int)j0;
-      int32_t ntn = 0xffff;
-      for (;;) {
-        atomicAdd(&tile[rowbase + e.x], (int)(x * __int_as_float(e.y)));
-        ++my_products;
-        ++my_cursor;
-        if (++cur >= r.w) break;
-        e = a.S.ent[cur];
-        const int32_t tn = (int32_t)a.S.tile[cur];
-        if (tn != J) { ntn = tn; break; }
-      }
-      if (k < CAP_SI) { s_nt[k] = (uint16_t)ntn; s_cur[k] = cur; }
-      else { a.snext_spill[sb + k] = ntn; a.scur[sb + k] = cur; }
-    }
-    if (hk >= 0) {  // some cursors moved: recompute this thread's next due tile
-      my_due = 0xffff;
-      for (int k = tl; k < nS; k += SCAN_THREADS) {
-        const int32_t nt = k < CAP_SI ? (int32_t)s_nt[k] : a.snext_spill[sb + k];
-        my_due = nt < my_due ? nt : my_due;
+    // ---- 1b. cursor tier: add this tile's bucket -----------------------------
+    if (k0 < pe) {
+      atomicAdd(&tile[hp.x], hp.y);
+      for (int32_t k = k0 + SCAN_THREADS; k < pe; k += SCAN_THREADS) {
+        const int2 q = cprod[k];
+        atomicAdd(&tile[q.x], q.y);
       }
     }
     __syncthreads();
-
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
 #pragma unroll 1
@@ -405,7 +496,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int tot = __popcll(me);
         const int leader = __ffsll((long long)me) - 1;
         unsigned int lb = 0;
-        if (lane == leader) lb = atomicAdd(&obuf_n, (unsigned)tot);
+        if (lane_id() == leader) lb = atomicAdd(&obuf_n, (unsigned)tot);
         lb = __shfl(lb, leader);
         apss_pair pr;
         pr.i = (uint32_t)(i0 + il + a.i_off);
@@ -442,14 +533,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       __syncthreads();
     }
   }
+  __syncthreads();
+  }  // band queue
 
-  atomicAdd(&acc_products, (unsigned long long)my_products);
+  atomicAdd(&acc_products, (unsigned long long)my_products + my_cursor);
   atomicAdd(&acc_head, (unsigned long long)my_head);
   atomicAdd(&acc_cells, (unsigned long long)my_cells);
   atomicAdd(&acc_visits, (unsigned long long)my_visits);
   atomicAdd(&acc_cursor, (unsigned long long)my_cursor);
   __syncthreads();
-  if (tid == 0) {
+  if (tid0 == 0) {
     atomicAdd(&a.stats[0], acc_products);
     atomicAdd(&a.stats[1], acc_head);
     atomicAdd(&a.stats[2], acc_cells);
