@@ -22,6 +22,7 @@ HOST_LIB_PATH = os.path.join(HERE, "libapss_host.so")
 
 APSS_OK, APSS_E_INVALID, APSS_E_NOMEM, APSS_E_DEVICE, APSS_E_UNSUPPORTED, APSS_E_CANCELLED, APSS_E_BUSY = (
     0, -1, -2, -3, -4, -5, -6)
+APSS_PREC_FP32, APSS_PREC_BF16_HEAD = 0, 1
 PREC_FP32, PREC_BF16_HEAD = 0, 1
 
 PAIR_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("score", "<f4")])
@@ -37,14 +38,14 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", c_int32), ("precision", c_int32), ("sort_output", c_int32),
                 ("rescore_all", c_int32), ("output_on_device", c_int32),
                 ("dense_features", c_int32), ("mem_budget", c_int64), ("stream", c_void_p),
-                ("reserved_", c_int64 * 4)]
+                ("mfma_head", c_int32), ("reserved32_", c_int32), ("reserved_", c_int64 * 3)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("count", c_int64), ("pairs", c_void_p), ("on_device", c_int32),
                 ("p_used", c_int32), ("cells", c_int64), ("band", c_int64),
                 ("products", c_int64), ("head_pairs", c_int64), ("list_visits", c_int64),
-                ("cursor_products", c_int64),
+                ("cursor_products", c_int64), ("mfma_features", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
                 ("internal_", c_void_p)]

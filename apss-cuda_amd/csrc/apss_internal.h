@@ -23,9 +23,12 @@ namespace apss {
 // row holds at most KD_MAX <= CAP_J bitmap-tier entries, a single row fits.
 constexpr int BAND_ROWS = 128;
 constexpr int TILE_ROWS = 64;
-constexpr int TSTRIDE = TILE_ROWS + 1;
+// Row stride of the score tile: 68 words puts the four row groups that one
+// wave reads in the MFMA output layout (rows 4 apart, 16 consecutive columns
+// each) in disjoint bank ranges.
+constexpr int TSTRIDE = 68;
 constexpr int SCAN_THREADS = 1024;           // band kernel workgroup (16 waves)
-constexpr int KD_MAX = 4096;                 // features in the bitmap tier
+constexpr int KD_MAX = 4032;                 // features in the bitmap tier (KD + 1 <= 4 x 1024)
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
 constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
@@ -33,10 +36,31 @@ constexpr int CAP_S = 8192;                  // cursor-tier records of a band (b
 constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
 constexpr int CH_TILES = 1024;               // J tiles per cursor-bucket chunk
 
+// Dense MFMA head: the most frequent tail features, stored per row as a
+// 256-byte record and contracted on the matrix cores.
+//   DENSE_I8   64 features, each value as a 28-bit fixed-point integer split in
+//              four signed 8-bit limbs (record = limb-major, 4 x 64 bytes);
+//              integer MFMA makes the head exact up to the fixed-point step.
+//   DENSE_BF16 128 features as BF16 (round to nearest even), FP32 MFMA
+//              accumulation; pairs within the BF16 error bound are rescored.
+enum : int32_t { DENSE_OFF = 0, DENSE_I8 = 1, DENSE_BF16 = 2 };
+constexpr int DENSE_ROW_BYTES = 256;
+constexpr int DENSE_I8_FEATURES = 64;
+constexpr int DENSE_BF16_FEATURES = 128;
+// With a dense head the sparse blocks are smaller, and the LDS they free holds
+// the head operands: caps of the bitmap tier per band / tile / row.
+constexpr int CAP_I_DENSE = 6144;
+constexpr int CAP_J_DENSE = 3072;
+constexpr int KD_MAX_DENSE = 3072;
+__host__ __device__ constexpr int cap_i_for(int dm) { return dm ? CAP_I_DENSE : CAP_I; }
+__host__ __device__ constexpr int cap_j_for(int dm) { return dm ? CAP_J_DENSE : CAP_J; }
+__host__ __device__ constexpr int kd_max_for(int dm) { return dm ? KD_MAX_DENSE : KD_MAX; }
+
 // Feature classes in fmap[] (values >= 0 are bitmap-tier indices).
 constexpr int32_t F_HEAD = -1;    // id < p: head (forward list / MFMA)
 constexpr int32_t F_CURSOR = -2;  // short list: cursor tier
 constexpr int32_t F_IGNORE = -3;  // cannot form a pair (list shorter than needed)
+constexpr int32_t F_DENSE0 = -16; // F_DENSE0 - k: k-th feature of the dense MFMA head
 
 // Error bits raised by the validation kernel.
 enum : uint32_t {
@@ -89,6 +113,13 @@ struct TileSide {
   // cursor-tier entries of this side (one record per entry, grouped by tile)
   const int64_t* sbase = nullptr; // T+1
   const int4* srec = nullptr;     // {local row, float bits of x, first cursor, end}
+  // bitmap tier, per tile: start of every feature's segment in the tile
+  // block, indexed by bitmap index (KD + 1 entries, row stride kds)
+  const uint16_t* dseg = nullptr;
+  int32_t kds = 0;
+  // dense MFMA head: one 256-byte record per row (16 x 16-byte chunks)
+  const uint4* dense = nullptr;
+  const float* dnorm = nullptr;   // norm of the head part of each row, rounded up (BF16)
 };
 
 // Cursor-tier inverted lists of the corpus side: {row, float bits of value},
@@ -128,13 +159,18 @@ struct ScanArgs {
   const int32_t* band_jend;  // per band: one past its last partner tile (NULL: all)
   const double* band_t;      // per band: threshold (NULL: t)
   double eps_max;            // largest cell error bound (for per-band fast rejects)
-  int32_t* scur;             // cursor position of every cursor record (I side)
-  int32_t* snext;            // J tile the cursor points into (INT32_MAX: exhausted)
+  int4* scs;                 // per cursor record: {position, end, J tile at position
+                             // (INT32_MAX: exhausted), unused}
   // cursor-tier products, bucketed by J tile for one chunk of CH_TILES tiles;
   // one region of cprod_cap records per workgroup of the persistent grid
   int2* cprod;               // {cell = il * TSTRIDE + jl, fixed-point product}
   int64_t cprod_cap;
   unsigned long long* band_next;  // band queue of the persistent grid
+  // dense head: int8 limb groups combine as hi << dsh_hi + (lo >> dsh_lo);
+  // eps_dense bounds the int8 head's error, c_bf * |x_D||y_D| the BF16 head's
+  int32_t dsh_hi, dsh_lo;
+  double eps_dense;
+  double c_bf;
 };
 
 // Arguments of the per-tile bitmap-tier build kernels.
@@ -162,6 +198,8 @@ struct BuildSide {
   float* dval;
   uint16_t* dfeat;
   int4* srec;
+  uint16_t* dseg;             // T*kds segment starts by bitmap index
+  int32_t kds;
   const int64_t* sptr;        // corpus cursor lists
   const int2* sent;
 };

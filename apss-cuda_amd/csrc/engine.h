@@ -26,8 +26,12 @@ __global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
 __global__ void k_tile_compact(BuildSide s);
 __global__ void k_row2tile(const int64_t* tstart, int32_t* row2tile);
+__global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, const float* val,
+                             int32_t p, const int32_t* fmap, int32_t dm, int32_t B, uint32_t* dense,
+                             float* dnorm);
 __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
                              const int64_t* tstart, uint32_t* spos);
+template <int DM>
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
@@ -70,6 +74,8 @@ struct Tiers {
   int32_t p = 0;
   int32_t KD = 64, KW = 1;
   int32_t n_bitmap = 0;          // features in the bitmap tier
+  int32_t dm = DENSE_OFF;        // dense MFMA head mode
+  int32_t n_dense = 0;           // features in the dense head
   int64_t n_cursor_entries = 0;
   int32_t* fmap = nullptr;       // d
   int64_t* sptr = nullptr;       // d+1
@@ -87,6 +93,10 @@ struct Side {
   float* hnorm = nullptr;
   int32_t maxlen = 0;
   float maxnorm = 0.f;
+  // dense head records (n x 256 bytes) for one tiering of the corpus
+  uint4* dense = nullptr;
+  float* dnorm = nullptr;        // BF16 head: norm of each row's head part
+  int32_t dense_B = 0;           // int8 head: values are scaled by 2^dense_B
 };
 
 // Per-tile structures of one tiling of a side.
@@ -104,6 +114,8 @@ struct TileSet {
   uint16_t* dfeat = nullptr;
   int64_t* sbase = nullptr;
   int4* srec = nullptr;
+  uint16_t* dseg = nullptr;      // T x kds segment starts by bitmap index
+  int32_t kds = 0;
   int64_t nD = 0, nS = 0;
   int64_t max_s = 0;             // most cursor records in one tile
   std::vector<int64_t> cuts;     // host copy of tstart
@@ -136,6 +148,7 @@ struct ScanOutput {
   apss_pair* pairs = nullptr;   // device, owned by the arena
   int64_t count = 0;
   int64_t band = 0, products = 0, head_pairs = 0, cells = 0, visits = 0, cursor = 0;
+  int32_t n_dense = 0;
   double ms_scan = 0, ms_rescore = 0;
 };
 
@@ -144,7 +157,12 @@ void check_csr_host(const apss_csr& X, int32_t p);
 // Upload (if needed) a CSR matrix and validate it; computes norms.
 Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p);
 // Histogram + tier selection + cursor lists of a corpus side.
-Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap);
+Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap,
+                  int32_t dm);
+// Dense-head records of a side for the tiers' head features (no-op without a head).
+void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr);
+// Dense-head mode asked for by the options.
+int32_t dense_mode(const apss_opts& o);
 // Per-tile bitmap tier (+ cursor records when `records`) of a side against
 // corpus tiers, for tiles of at most max_rows rows and cap_d bitmap entries.
 TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, int32_t self,

@@ -116,6 +116,10 @@ TileSide make_view(const Side& s, const TileSet& ts) {
   v.dfeat = ts.dfeat;
   v.sbase = ts.sbase;
   v.srec = ts.srec;
+  v.dseg = ts.dseg;
+  v.kds = ts.kds;
+  v.dense = s.dense;
+  v.dnorm = s.dnorm;
   return v;
 }
 
@@ -205,7 +209,13 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
 // ---------------------------------------------------------------------------
 // build_tiers: histogram, class per feature, cursor lists
 
-Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap) {
+int32_t dense_mode(const apss_opts& o) {
+  if (o.mfma_head < 0) return DENSE_OFF;
+  return o.precision == APSS_PREC_BF16_HEAD ? DENSE_BF16 : DENSE_I8;
+}
+
+Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap,
+                  int32_t dm) {
   Tiers tr;
   tr.d = s.d;
   tr.p = p;
@@ -225,22 +235,30 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 
   const int32_t need = self ? 2 : 1;
   // bitmap tier: the lists present in a sizeable share of the 64-row tiles,
-  // plus every list too long for the cursor tier
+  // plus every list too long for the cursor tier; the most frequent of them
+  // form the dense MFMA head when there is one
   const int64_t T = std::max<int64_t>(1, (s.n + TILE_ROWS - 1) / TILE_ROWS);
   const int64_t lmin = std::max<int64_t>(need, T / 8);
-  int32_t cap = kd_cap > 0 ? std::min(kd_cap, KD_MAX) : KD_MAX;
   std::vector<int32_t> cand;
-  int64_t n_forced = 0;
-  for (int64_t c = p; c < d; ++c) {
+  for (int64_t c = p; c < d; ++c)
     if (h[c] >= lmin || h[c] > S_LIST_MAX) cand.push_back((int32_t)c);
-    if (h[c] > S_LIST_MAX) ++n_forced;
-  }
+  std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) { return h[x] > h[y]; });
+  const int32_t slots = dm == DENSE_I8 ? DENSE_I8_FEATURES : dm == DENSE_BF16 ? DENSE_BF16_FEATURES : 0;
+  const int32_t n_dense = std::min<int32_t>(slots, (int32_t)cand.size());
+  if (n_dense == 0) dm = DENSE_OFF;
+  std::vector<int32_t> dense(cand.begin(), cand.begin() + n_dense);
+  cand.erase(cand.begin(), cand.begin() + n_dense);
+  const int32_t kdm = kd_max_for(dm);
+  int32_t cap = kd_cap > 0 ? std::min(kd_cap, kdm) : kdm;
+  int64_t n_forced = 0;
+  for (int32_t c : cand) n_forced += h[c] > S_LIST_MAX;
   if (n_forced > cap)
     throw Error{APSS_E_UNSUPPORTED, std::to_string(n_forced) + " features have lists longer than " +
                                         std::to_string(S_LIST_MAX) + " but the bitmap tier holds " +
                                         std::to_string(cap)};
-  std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) { return h[x] > h[y]; });
   if ((int64_t)cand.size() > cap) cand.resize(cap);
+  tr.dm = dm;
+  tr.n_dense = n_dense;
   tr.n_bitmap = (int32_t)cand.size();
   tr.KD = std::max(64, (tr.n_bitmap + 63) / 64 * 64);
   tr.KW = tr.KD / 64;
@@ -252,6 +270,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   for (int64_t c = p; c < d; ++c)
     if (h[c] >= need) fm[c] = F_CURSOR;
   for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
+  for (size_t k = 0; k < dense.size(); ++k) fm[dense[k]] = F_DENSE0 - (int32_t)k;
   std::vector<int64_t> sp((size_t)d + 1, 0);
   std::vector<int32_t> small_lists, large_lists;
   for (int64_t c = 0; c < d; ++c) {
@@ -419,6 +438,11 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   b.dfeat = ts.dfeat;
   b.cst = ts.cst;
   b.srec = records ? ts.srec : nullptr;
+  // row stride: KD + 1 starts rounded up to 4 (8-byte rows for the staging loads)
+  ts.kds = (tr.KD + 1 + 3) / 4 * 4;
+  ts.dseg = ar.alloc<uint16_t>((int64_t)std::max(T, 1) * ts.kds);
+  b.dseg = ts.dseg;
+  b.kds = ts.kds;
   if (T > 0) {
     hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_tile_compact, dim3(T), dim3(256), 0, st, b);
@@ -439,6 +463,22 @@ void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles
   }
 }
 
+void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
+  s.dense = nullptr;
+  s.dnorm = nullptr;
+  if (tr.dm == DENSE_OFF) return;
+  // int8 head: |x| <= maxnorm, so x * 2^B stays within 2^28 (four limbs)
+  const double m = std::max((double)s.maxnorm, 1e-30);
+  s.dense_B = 28 - (int)std::ceil(std::log2(m));
+  s.dense = ar.alloc<uint4>(std::max<int64_t>(s.n, 1) * 16);
+  if (tr.dm == DENSE_BF16) s.dnorm = ar.alloc<float>(s.n);
+  if (s.n > 0) {
+    hipLaunchKernelGGL(k_dense_rows, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n, s.rp,
+                       s.col, s.val, tr.p, tr.fmap, tr.dm, s.dense_B, (uint32_t*)s.dense, s.dnorm);
+    APSS_HIP(hipGetLastError());
+  }
+}
+
 // ---------------------------------------------------------------------------
 // scan + rescore
 
@@ -453,7 +493,30 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   S = std::max(-100, std::min(S, 100));
   const double qscale = std::ldexp(1.0, S), qinv = std::ldexp(1.0, -S);
   const double head_eps = tr.p > 0 ? gamma32(maxlen + 1) * mI * mJ : 0.0;
-  const double eps_max = cell_error_bound((int)maxlen, mI, mJ, qinv, head_eps);
+  // dense head bounds.  int8: quantisation |x - X 2^-B| <= 2^-(B+1) per value,
+  // summed with ||x_H||_1 <= 8 ||x||; the dropped limb groups (weights 2^8
+  // and 2^0, <= 64 * (2 * 2^14 * 2^8 + 2^14)); one fixed-point step for the
+  // truncation of the combined value.  BF16: RNE per value (u = 2^-8, so
+  // 2^-7 + 2^-16 per product), FP32 accumulation of 128 exact products
+  // (gamma_128, doubled as a margin for the MFMA summation order), one step.
+  int32_t dsh_hi = 0, dsh_lo = 0;
+  double eps_dense = 0.0, c_bf = 0.0, eps_dense_max = 0.0;
+  if (tr.dm == DENSE_I8) {
+    const int E = I.dense_B + J.dense_B - S;  // the int8 value v has weight 2^-(BI+BJ); the tile 2^-S
+    if (E < 16 || E > 32)
+      throw Error{APSS_E_UNSUPPORTED, "values too small for the int8 MFMA head (set mfma_head = -1)"};
+    dsh_hi = 32 - E;
+    dsh_lo = E - 16;
+    const double bI = std::ldexp(1.0, -I.dense_B), bJ = std::ldexp(1.0, -J.dense_B);
+    eps_dense = (8.0 * mI * 0.5 * bJ + 8.0 * mJ * 0.5 * bI + 64.0 * 0.25 * bI * bJ +
+                 64.0 * (std::ldexp(1.0, 23) + std::ldexp(1.0, 14)) * bI * bJ + qinv) * (1.0 + 1e-9);
+    eps_dense_max = eps_dense;
+  } else if (tr.dm == DENSE_BF16) {
+    c_bf = (std::ldexp(1.0, -7) + std::ldexp(1.0, -16)) +
+           2.0 * gamma32(DENSE_BF16_FEATURES) * (1.0 + std::ldexp(1.0, -7) + std::ldexp(1.0, -16));
+    eps_dense_max = c_bf * mI * mJ + qinv;
+  }
+  const double eps_max = cell_error_bound((int)maxlen, mI, mJ, qinv, head_eps) + eps_dense_max;
   // fast-reject bound: the smallest threshold any band uses
   const double t_min = rc.band_t ? rc.t_min : rc.t;
   const double qlo = std::floor((t_min - 2.0 * eps_max) * qscale);
@@ -467,8 +530,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   cap = std::max<int64_t>(cap, 1 << 16);
   int64_t bcap = std::max<int64_t>(1 << 20, cap / 64);
   unsigned long long* ctr = ar.alloc<unsigned long long>(8);
-  int32_t* scur = ar.alloc<int32_t>(bands.nS);
-  int32_t* snext = ar.alloc<int32_t>(bands.nS);
+  int4* scs = ar.alloc<int4>(bands.nS);
   // persistent grid: one 1024-thread workgroup per CU pulls bands from a queue
   // and owns one cursor-product region, large enough for every product one
   // tile pair can receive (records x tile rows)
@@ -515,14 +577,24 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.band_first = 0;
     a.band_count = bands.T;
     a.stats = ctr + 2;
-    a.scur = scur;
-    a.snext = snext;
+    a.scs = scs;
     a.cprod = cprod;
     a.cprod_cap = cprod_cap;
     a.band_next = ctr + 7;
+    a.dsh_hi = dsh_hi;
+    a.dsh_lo = dsh_lo;
+    a.eps_dense = eps_dense;
+    a.c_bf = c_bf;
+    // BF16 head: emitted scores carry the stated BF16 bound (pair set exact)
+    if (tr.dm == DENSE_BF16) a.score_rel = 1e300;
     tm.start();
     if (bands.T > 0 && tiles.T > 0) {
-      hipLaunchKernelGGL(k_band_scan, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
+      if (tr.dm == DENSE_I8)
+        hipLaunchKernelGGL(k_band_scan<DENSE_I8>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
+      else if (tr.dm == DENSE_BF16)
+        hipLaunchKernelGGL(k_band_scan<DENSE_BF16>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
+      else
+        hipLaunchKernelGGL(k_band_scan<DENSE_OFF>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
       APSS_HIP(hipGetLastError());
     }
     so.ms_scan = tm.stop_ms();
@@ -543,6 +615,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.visits = (int64_t)h[5];
     so.cursor = (int64_t)h[6];
     so.band = nband;
+    so.n_dense = tr.n_dense;
     if (nout + nband > cap) {  // room for every band pair before the rescore
       apss_pair* bigger = ar.alloc<apss_pair>(nout + nband);
       if (nout) APSS_HIP(hipMemcpyAsync(bigger, out, sizeof(apss_pair) * nout, hipMemcpyDeviceToDevice, st));
@@ -609,10 +682,11 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   Side s = prepare_side(ar, st, *X, p);
   out->ms_upload = tm.stop_ms();
   tm.start();
-  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_S, true, {});
-  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, {});
+  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features, dense_mode(*o));
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, cap_i_for(tr.dm), CAP_S, true, {});
+  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, cap_j_for(tr.dm), INT32_MAX, false, {});
   set_cursor_tiles(ar, st, tr, tiles);
+  build_dense(ar, st, s, tr);
   out->ms_index = tm.stop_ms();
   RunConfig rc;
   rc.t = t;
@@ -630,6 +704,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->head_pairs = so.head_pairs;
   out->list_visits = so.visits;
   out->cursor_products = so.cursor;
+  out->mfma_features = so.n_dense;
   tm.start();
   if (o->output_on_device && !o->sort_output) {
     out->pairs = (apss_pair*)ar.detach(so.pairs);
@@ -743,13 +818,14 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   Side s = prepare_side(ar, st, all, p);
   const double ms_upload = tm.stop_ms();
   tm.start();
-  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features);
+  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features, dense_mode(*o));
   std::vector<int64_t> bounds(off.begin() + 1, off.end() - 1);
   std::sort(bounds.begin(), bounds.end());
   bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, CAP_I, CAP_S, true, bounds);
-  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, CAP_J, INT32_MAX, false, bounds);
+  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, cap_i_for(tr.dm), CAP_S, true, bounds);
+  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, cap_j_for(tr.dm), INT32_MAX, false, bounds);
   set_cursor_tiles(ar, st, tr, tiles);
+  build_dense(ar, st, s, tr);
   // per band: instance, partner range end, threshold
   std::vector<int32_t> hjend((size_t)std::max(bands.T, 1));
   std::vector<double> ht((size_t)std::max(bands.T, 1));

@@ -383,6 +383,71 @@ __global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {
     }
   }
   if (threadIdx.x == 0) crow[wp[s.KW]] = prow[s.KD];
+  // segment start of every feature (absent ones: empty segments), then the
+  // total in the remaining slots of the row
+  uint16_t* drow16 = s.dseg + (int64_t)tile * s.kds;
+  for (int32_t f = threadIdx.x; f < s.kds; f += blockDim.x)
+    drow16[f] = (uint16_t)prow[f < s.KD ? f : s.KD];
+}
+
+// ---------------------------------------------------------------------------
+// dense MFMA head records: one wave per row assembles the row's 256-byte
+// record in LDS (zero where the row lacks a head feature), then stores it.
+//   DENSE_I8   X = rint(x * 2^B), |X| <= 2^28, written as balanced base-256
+//              digits X = d0 2^24 + d1 2^16 + d2 2^8 + d3 (d1..d3 in
+//              [-128, 127]); limb a of head feature k at byte 64 a + k.
+//   DENSE_BF16 x rounded to BF16 (nearest even) at bytes 2k, 2k+1; dnorm[r]
+//              = norm of the row's head values, rounded up.
+
+__global__ __launch_bounds__(256) void k_dense_rows(int64_t n, const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ col,
+                                                    const float* __restrict__ val, int32_t p,
+                                                    const int32_t* __restrict__ fmap, int32_t dm,
+                                                    int32_t B, uint32_t* __restrict__ dense,
+                                                    float* __restrict__ dnorm) {
+  __shared__ uint32_t rec[4][DENSE_ROW_BYTES / 4];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  uint8_t* rb = reinterpret_cast<uint8_t*>(rec[w]);
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t r = blockIdx.x * 4 + w; r < n; r += waves) {
+    rec[w][lane] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double ss = 0.0;
+    for (int64_t e = rp[r] + lane; e < rp[r + 1]; e += 64) {
+      const int32_t c = col[e];
+      if (c < p) continue;
+      const int32_t f = fmap[c];
+      if (f > F_DENSE0) continue;
+      const int k = F_DENSE0 - f;
+      const float x = val[e];
+      if (dm == DENSE_I8) {
+        long long X = llrint(ldexp((double)x, B));
+#pragma unroll
+        for (int a = 3; a >= 1; --a) {
+          const int dgt = (int)(((X + 128) & 255) - 128);
+          rb[64 * a + k] = (uint8_t)(int8_t)dgt;
+          X = (X - dgt) >> 8;
+        }
+        rb[k] = (uint8_t)(int8_t)X;
+      } else {
+        uint32_t u = __float_as_uint(x);
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+        rb[2 * k] = (uint8_t)((u >> 16) & 0xffu);
+        rb[2 * k + 1] = (uint8_t)(u >> 24);
+        ss += (double)x * (double)x;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    dense[r * (DENSE_ROW_BYTES / 4) + lane] = rec[w][lane];
+    if (dm == DENSE_BF16) {
+      for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+      if (lane == 0) dnorm[r] = round_up_f(sqrt(ss) * (1.0 + 1e-15));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // J tile and row within that tile of every cursor-list entry, packed as

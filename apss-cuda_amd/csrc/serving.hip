@@ -93,9 +93,11 @@ static std::unique_ptr<Segment> build_segment(HostCSR&& h, int32_t p, const apss
   StreamScope ss(nullptr, o.device);
   apss_csr v = sg->host.view();
   sg->side = prepare_side(*sg->ar, ss.s, v, p);
-  sg->tr = build_tiers(*sg->ar, ss.s, sg->side, p, 0, o.dense_features);
-  sg->tiles = build_tiles(*sg->ar, ss.s, sg->side, sg->tr, 0, TILE_ROWS, CAP_J, INT32_MAX, false, {});
+  sg->tr = build_tiers(*sg->ar, ss.s, sg->side, p, 0, o.dense_features, dense_mode(o));
+  sg->tiles = build_tiles(*sg->ar, ss.s, sg->side, sg->tr, 0, TILE_ROWS, cap_j_for(sg->tr.dm),
+                          INT32_MAX, false, {});
   set_cursor_tiles(*sg->ar, ss.s, sg->tr, sg->tiles);
+  build_dense(*sg->ar, ss.s, sg->side, sg->tr);
   APSS_HIP(hipStreamSynchronize(ss.s));
   return sg;
 }
@@ -121,7 +123,10 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
   for (double x : t_req) t_min = std::min(t_min, x);
   for (const auto& sg : ix.segs) {
     tm.start();
-    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, CAP_I, CAP_S, true, bounds);
+    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, cap_i_for(sg->tr.dm), CAP_S, true,
+                                bounds);
+    Side qd = qs;  // the query's head records follow this segment's head features
+    build_dense(ar, st, qd, sg->tr);
     std::vector<double> ht((size_t)std::max(bands.T, 1), t_min);
     for (int32_t b = 0; b < bands.T; ++b) {
       const int64_t r = bands.cuts[b];
@@ -140,7 +145,7 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     rc.j_off = sg->row_off;
     rc.band_t = dt;
     rc.t_min = t_min;
-    ScanOutput so = scan_and_rescore(ar, st, qs, bands, sg->side, sg->tiles, sg->tr, rc);
+    ScanOutput so = scan_and_rescore(ar, st, qd, bands, sg->side, sg->tiles, sg->tr, rc);
     const size_t at = pairs.size();
     pairs.resize(at + (size_t)so.count);
     if (so.count)
@@ -159,6 +164,9 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     }
     ar.release(so.pairs);
     ar.release(dt);
+    if (qd.dense) ar.release(qd.dense);
+    if (qd.dnorm) ar.release(qd.dnorm);
+    if (stats) stats->mfma_features = so.n_dense;
   }
   if (stats) stats->ms_upload = ms_up;
 }

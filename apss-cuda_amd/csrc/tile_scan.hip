@@ -53,47 +53,78 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 // are issued into registers before the current tile's product phase and
 // written to LDS after it, so their latency hides behind the products.
 constexpr int OBUF_CAP = 512;               // LDS output buffer (pairs)
-constexpr int I_PER_THREAD = CAP_I / SCAN_THREADS;
-constexpr int CST_PAIRS = (KD_MAX + 2 + 2 * SCAN_THREADS - 1) / (2 * SCAN_THREADS);
-constexpr int QUADS_PER_THREAD = (CAP_J / 4 + SCAN_THREADS - 1) / SCAN_THREADS;
 static_assert(KD_MAX < 65536 && CAP_J < 65536, "16-bit segment starts");
 
-struct Prefetch {
-  uint64_t bmp;
-  uint32_t wpre;
-  uint32_t cst[CST_PAIRS];  // two 16-bit segment starts per register
-  uint32_t rows[QUADS_PER_THREAD];
-  float4 vals[QUADS_PER_THREAD];
-  int32_t ncs, nd4, nrows;
+// Per-mode sizes: bitmap-tier entries per thread of a band and of a tile.
+template <int DM>
+struct Sizes {
+  static constexpr int CI = cap_i_for(DM), CJ = cap_j_for(DM), KDM = kd_max_for(DM);
+  static constexpr int IPT = CI / SCAN_THREADS;
+  static constexpr int QPT = (CJ / 4 + SCAN_THREADS - 1) / SCAN_THREADS;
+  static_assert(CI % SCAN_THREADS == 0 && IPT % 2 == 0, "band entries split in two halves");
+  static_assert(KDM + 1 <= 4 * SCAN_THREADS, "one 4-entry chunk of segment starts per thread");
 };
 
-__device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int KW, int tid,
-                                              Prefetch& pf) {
-  const int64_t c0 = S.cbase[t];
-  pf.ncs = (int)(S.cbase[t + 1] - c0);  // present features + 1
+template <int DM>
+struct Prefetch {
+  uint2 seg;                      // four 16-bit segment starts
+  uint32_t rows[Sizes<DM>::QPT];
+  float4 vals[Sizes<DM>::QPT];
+  uint4 dense;                    // one 16-byte chunk of the tile's head records
+  uint32_t aux;                   // one row value: norm, head norm, length or head-part norm
+  int32_t nd4, nrows;
+};
+
+template <int DM>
+__device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int tid,
+                                              Prefetch<DM>& pf) {
   const int64_t d0 = S.dbase[t];
   pf.nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
-  pf.nrows = (int)(S.tstart[t + 1] - S.tstart[t]);
-  if (tid < KW) {
-    pf.bmp = S.bmp[(int64_t)t * KW + tid];
-    pf.wpre = S.wpre[(int64_t)t * KW + tid];
-  }
+  const int64_t r0 = S.tstart[t];
+  pf.nrows = (int)(S.tstart[t + 1] - r0);
+  if (4 * tid < S.kds) pf.seg = reinterpret_cast<const uint2*>(S.dseg + (int64_t)t * S.kds)[tid];
 #pragma unroll
-  for (int i = 0; i < CST_PAIRS; ++i) {
-    const int k = 2 * (tid + i * SCAN_THREADS);
-    uint32_t lo = k < pf.ncs ? (uint32_t)S.cst[c0 + k] : 0u;
-    uint32_t hi = k + 1 < pf.ncs ? (uint32_t)S.cst[c0 + k + 1] : 0u;
-    pf.cst[i] = lo | (hi << 16);
-  }
-#pragma unroll
-  for (int i = 0; i < QUADS_PER_THREAD; ++i) {
+  for (int i = 0; i < Sizes<DM>::QPT; ++i) {
     const int k = tid + i * SCAN_THREADS;
     if (k < pf.nd4) {
       pf.rows[i] = reinterpret_cast<const uint32_t*>(S.drow + d0)[k];
       pf.vals[i] = reinterpret_cast<const float4*>(S.dval + d0)[k];
     }
   }
+  if constexpr (DM != DENSE_OFF) {  // thread -> (row tid / 16, chunk tid % 16)
+    const int row = tid >> 4;
+    pf.dense = row < pf.nrows ? S.dense[(r0 + row) * 16 + (tid & 15)] : make_uint4(0, 0, 0, 0);
+  }
+  // per-row values, one per thread of the first four waves
+  {
+    const int row = tid & 63, which = tid >> 6;
+    uint32_t v = 0;
+    if (row < pf.nrows) {
+      const int64_t r = r0 + row;
+      if (which == 0) v = __float_as_uint(S.norm[r]);
+      else if (which == 1) v = __float_as_uint(S.hnorm[r]);
+      else if (which == 2) v = (uint32_t)(S.rp[r + 1] - S.rp[r]);
+      else if (which == 3 && DM == DENSE_BF16) v = __float_as_uint(S.dnorm[r]);
+    }
+    pf.aux = v;
+  }
 }
+
+// Workgroup barrier for hand-offs through LDS only: it waits for this wave's
+// LDS operations but not for its global loads and stores in flight (the
+// prefetch of the next tile, output stores), which __syncthreads() would.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Head records sit in LDS as 16 chunks of 16 bytes per row, chunk c of row r
+// at slot c ^ (r % 16): the 16 rows one MFMA operand load touches then spread
+// over the banks.
+__device__ __forceinline__ int dslot(int row, int chunk) { return row * 16 + (chunk ^ (row & 15)); }
 
 // Exclusive scan over the workgroup (one value per thread); *total gets the
 // sum.  wsum holds 17 words of LDS.
@@ -124,65 +155,77 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, int tl, uint32_t* w
 
 // Cursor walks over one record's list from position cur, visiting the entries
 // whose J tile lies below ce (lists are row-sorted, so those come first).
-// Positions are read four at a time so one memory round trip covers up to
-// four entries.
+// Positions are read eight at a time so one memory round trip covers up to
+// eight entries.
+constexpr int WALK_BATCH = 8;
+
 __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
                                            int32_t cs, int32_t ce, uint32_t* bkt) {
   for (;;) {
-    uint32_t pt[4];
+    uint32_t pt[WALK_BATCH];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pt[q] = cur + q < end ? pos[cur + q] : 0xffffffffu;
+    for (int q = 0; q < WALK_BATCH; ++q) pt[q] = cur + q < end ? pos[cur + q] : 0xffffffffu;
     bool go = true;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < WALK_BATCH; ++q) {
       const int32_t t = (int32_t)(pt[q] >> 8);
       go = go && t < ce;
       if (go) atomicAdd(&bkt[t - cs], 1u);
     }
     if (!go) return;
-    cur += 4;
+    cur += WALK_BATCH;
   }
 }
 
+// Writes the products and returns the first position not consumed; *t_next
+// receives the J tile of that position (INT32_MAX at the list end).
 __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
                                             const int2* __restrict__ ent, int cur, int end,
                                             int32_t cs, int32_t ce, int rowc, float x,
                                             uint32_t* bfill, int2* __restrict__ cprod,
-                                            unsigned int& n_prod) {
+                                            unsigned int& n_prod, int32_t* t_next) {
   for (;;) {
-    uint32_t pt[4];
-    int2 e[4];
+    uint32_t pt[WALK_BATCH];
+    int2 e[WALK_BATCH];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < WALK_BATCH; ++q) {
       const bool ok = cur + q < end;
       pt[q] = ok ? pos[cur + q] : 0xffffffffu;
       e[q] = ok ? ent[cur + q] : make_int2(0, 0);
     }
     bool go = true;
     int used = 0;
+    int32_t stop_t = INT32_MAX;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < WALK_BATCH; ++q) {
       const int32_t t = (int32_t)(pt[q] >> 8);
+      const bool was = go;
       go = go && t < ce;
       if (go) {
         const uint32_t slot = atomicAdd(&bfill[t - cs], 1u);
         cprod[slot] = make_int2(rowc + (int)(pt[q] & 0xffu), (int)(x * __int_as_float(e[q].y)));
         ++used;
+      } else if (was) {
+        stop_t = pt[q] == 0xffffffffu ? INT32_MAX : t;
       }
     }
     cur += used;
     n_prod += (unsigned)used;
-    if (!go) return cur;
+    if (!go) {
+      *t_next = stop_t;
+      return cur;
+    }
   }
 }
 
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
+// DM selects the dense head (DENSE_OFF, DENSE_I8, DENSE_BF16).
+template <int DM>
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
+  using Z = Sizes<DM>;
   __shared__ int32_t tile[BAND_ROWS * TSTRIDE];     // fixed-point scores (x 2^S)
-  __shared__ uint64_t bJ[KW_MAX];
-  __shared__ uint16_t wpJ[KW_MAX];
-  __shared__ uint16_t cJ[KD_MAX + 1];
-  __shared__ __align__(16) int2 eJ[CAP_J];          // {local row, value bits}
+  __shared__ __align__(8) uint16_t cJ[4 * SCAN_THREADS];  // segment starts of tile J by bitmap index
+  __shared__ __align__(16) int2 eJ[Z::CJ];           // {local row, value bits}
   __shared__ uint32_t bkt[CH_TILES + 1];             // cursor-product bucket starts of the chunk
   __shared__ uint32_t bfill[CH_TILES];               // bucket fill pointers while scattering
   __shared__ uint32_t wsum[SCAN_THREADS / 64 + 1];
@@ -191,21 +234,27 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ unsigned long long obuf_base;
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
-  __shared__ float vI[CAP_I];                        // band entry values * 2^S, [q][tid]
+  __shared__ float vI[Z::CI];                        // band entry values * 2^S, [q][tid]
+  __shared__ uint4 dA[DM ? BAND_ROWS * 16 : 1];      // head records of the band
+  __shared__ uint4 dB[DM ? TILE_ROWS * 16 : 1];      // head records of tile J
+  __shared__ float dnI[DM == DENSE_BF16 ? BAND_ROWS : 1], dnJ[DM == DENSE_BF16 ? TILE_ROWS : 1];
   __shared__ int32_t band_s, J_first_s, ch_end_s;
+  __shared__ unsigned int cand_n;                    // candidate cells of the tile pair
+  __shared__ unsigned int rec_next;                  // cursor records handed out so far
   __shared__ unsigned long long acc_products, acc_head, acc_cells, acc_visits, acc_cursor;
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
+  static_assert(BAND_ROWS == 128 && TILE_ROWS == 64 && SCAN_THREADS == 1024,
+                "MFMA layout: 16 waves x two 16x16 blocks cover the 128 x 64 tile");
 
   const int tid0 = threadIdx.x;
-  const int KW = a.KW;
   int2* const cprod = a.cprod + (int64_t)blockIdx.x * a.cprod_cap;
 
   // the tile is cleared cell by cell by the threshold pass, so once is enough
   for (int c = tid0; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
-    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
+    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
   }
-  unsigned int my_products = 0, my_head = 0, my_cells = 0, my_visits = 0, my_cursor = 0;
+  unsigned int my_products = 0, my_visits = 0;
 
   for (;;) {
   int tid = tid0;
@@ -229,12 +278,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // Entry k = tid + q * SCAN_THREADS of the band block: (row * TSTRIDE,
   // bitmap index, value * 2^S).  The block is ordered by frequency rank, so
   // the lanes of a wave hold entries of the same or similar features.
-  int32_t e_rf[I_PER_THREAD];  // row * TSTRIDE << 12 | bitmap index (or -1)
+  int32_t e_rf[Z::IPT];  // row * TSTRIDE << 12 | bitmap index (or -1)
   {
     const int64_t d0 = a.I.dbase[I];
-    const int nE = a.I.cst[a.I.cbase[I + 1] - 1];  // entries (end sentinel)
+    const int nE = a.I.dseg[(int64_t)I * a.I.kds + a.KD];  // entries of the band block
 #pragma unroll
-    for (int q = 0; q < I_PER_THREAD; ++q) {
+    for (int q = 0; q < Z::IPT; ++q) {
       const int k = tid + q * SCAN_THREADS;
       const bool ok = k < nE;
       // bit 26 marks the first entry of a feature's band segment (it counts
@@ -251,6 +300,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     normI[tid] = ok ? a.I.norm[r] : 0.f;
     hnI[tid] = ok ? a.I.hnorm[r] : 0.f;
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
+    if constexpr (DM == DENSE_BF16) dnI[tid] = ok ? a.I.dnorm[r] : 0.f;
+  }
+  if constexpr (DM != DENSE_OFF) {  // the band's head records (zero beyond its rows)
+    for (int c = tid; c < BAND_ROWS * 16; c += SCAN_THREADS) {
+      const int row = c >> 4;
+      dA[dslot(row, c & 15)] = row < nI ? a.I.dense[(i0 + row) * 16 + (c & 15)] : make_uint4(0, 0, 0, 0);
+    }
   }
   if (tid == 0) {
     int32_t jf = 0;
@@ -269,14 +325,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   const int nS = (int)(a.I.sbase[I + 1] - sb);
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     const int4 r = a.I.srec[sb + k];
-    a.scur[sb + k] = r.z;
-    a.snext[sb + k] = r.z < r.w ? (int32_t)(a.S.pos[r.z] >> 8) : INT32_MAX;
+    a.scs[sb + k] = make_int4(r.z, r.w, r.z < r.w ? (int32_t)(a.S.pos[r.z] >> 8) : INT32_MAX, 0);
   }
   __syncthreads();
   const int32_t J_first = J_first_s;
 
-  Prefetch pf;
-  if (J_first < J_end) prefetch_tile(a.J, J_first, KW, tid, pf);
+  Prefetch<DM> pf;
+  if (J_first < J_end) prefetch_tile<DM>(a.J, J_first, tid, pf);
   int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
 
   for (int32_t J = J_first; J < J_end; ++J) {
@@ -288,19 +343,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // ---- 1. cursor tier: bucket the products of the next chunk of tiles ----
     // Every record walks its row-sorted list up to the chunk end twice: once
     // to count the products per J tile, once to write them (fixed point, with
-    // their cell) into this workgroup's region, grouped by tile.  A chunk
-    // whose products exceed the region ends early at a tile boundary.
+    // their cell) into this workgroup's region, grouped by tile.  Threads take
+    // records one at a time from a shared counter, so long lists spread over
+    // the workgroup.  A chunk whose products exceed the region ends early at a
+    // tile boundary.
     if (J == ch_hi) {
       ch_lo = J;
       const int32_t ce0 = J + CH_TILES < J_end ? J + CH_TILES : J_end;
       bkt[tl] = 0;
-      if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; }
+      if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; rec_next = 0; }
       __syncthreads();
-      for (int k = tl; k < nS; k += SCAN_THREADS) {
-        if (a.snext[sb + k] >= ce0) continue;
-        walk_count(a.S.pos, a.scur[sb + k], a.I.srec[sb + k].w, ch_lo, ce0, bkt);
+      for (;;) {
+        const int k = (int)atomicAdd(&rec_next, 1u);
+        if (k >= nS) break;
+        const int4 st = a.scs[sb + k];
+        if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt);
       }
       __syncthreads();
+      if (tl == 0) rec_next = 0;
       const uint32_t cnt = bkt[tl];
       uint32_t total;
       const uint32_t ex = block_exscan(cnt, tl, wsum, &total);
@@ -310,14 +370,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if ((int64_t)ex + cnt > a.cprod_cap) atomicMin(&ch_end_s, ch_lo + tl);
       __syncthreads();
       const int32_t ce = ch_end_s;
-      for (int k = tl; k < nS; k += SCAN_THREADS) {
-        if (a.snext[sb + k] >= ce) continue;
+      for (;;) {
+        const int k = (int)atomicAdd(&rec_next, 1u);
+        if (k >= nS) break;
+        const int4 st = a.scs[sb + k];
+        if (st.z >= ce) continue;
         const int4 r = a.I.srec[sb + k];
-        const int cur = walk_scatter(a.S.pos, a.S.ent, a.scur[sb + k], r.w, ch_lo, ce,
-                                     r.x * TSTRIDE, __int_as_float(r.y) * a.qscale, bfill, cprod,
-                                     my_cursor);
-        a.scur[sb + k] = cur;
-        a.snext[sb + k] = cur < r.w ? (int32_t)(a.S.pos[cur] >> 8) : INT32_MAX;
+        int32_t tn;
+        unsigned int np = 0;
+        const int cur = walk_scatter(a.S.pos, a.S.ent, st.x, st.y, ch_lo, ce, r.x * TSTRIDE,
+                                     __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn);
+        a.scs[sb + k] = make_int4(cur, st.y, tn, 0);
+        if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
       }
       __syncthreads();
       ch_hi = ce;
@@ -327,15 +391,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const int nJ = pf.nrows;
 
     // ---- write the prefetched tile J into LDS --------------------------------
-    if (tl < KW) { bJ[tl] = pf.bmp; wpJ[tl] = pf.wpre; }
+    if (4 * tl < a.J.kds) reinterpret_cast<uint2*>(cJ)[tl] = pf.seg;
 #pragma unroll
-    for (int i = 0; i < CST_PAIRS; ++i) {
-      const int k = 2 * (tl + i * SCAN_THREADS);
-      if (k < pf.ncs) cJ[k] = (uint16_t)(pf.cst[i] & 0xffffu);
-      if (k + 1 < pf.ncs) cJ[k + 1] = (uint16_t)(pf.cst[i] >> 16);
-    }
-#pragma unroll
-    for (int i = 0; i < QUADS_PER_THREAD; ++i) {
+    for (int i = 0; i < Z::QPT; ++i) {
       const int k = tl + i * SCAN_THREADS;
       if (k < pf.nd4) {
         const float vv[4] = {pf.vals[i].x, pf.vals[i].y, pf.vals[i].z, pf.vals[i].w};
@@ -344,18 +402,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
       }
     }
-    if (tl < TILE_ROWS) {
-      const bool ok = tl < nJ;
-      const int64_t r = j0 + tl;
-      normJ[tl] = ok ? a.J.norm[r] : 0.f;
-      hnJ[tl] = ok ? a.J.hnorm[r] : 0.f;
-      lenJ[tl] = ok ? (int32_t)(a.J.rp[r + 1] - a.J.rp[r]) : 0;
+    if (tl < 4 * TILE_ROWS) {
+      const int row = tl & 63, which = tl >> 6;
+      if (which == 0) normJ[row] = __uint_as_float(pf.aux);
+      else if (which == 1) hnJ[row] = __uint_as_float(pf.aux);
+      else if (which == 2) lenJ[row] = (int32_t)pf.aux;
+      else if (DM == DENSE_BF16) dnJ[row] = __uint_as_float(pf.aux);
     }
+    if constexpr (DM != DENSE_OFF) dB[dslot(tl >> 4, tl & 15)] = pf.dense;
 
-    __syncthreads();
+    lds_barrier();
 
     // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
-    if (J + 1 < J_end) prefetch_tile(a.J, J + 1, KW, tl, pf);
+    if (J + 1 < J_end) prefetch_tile<DM>(a.J, J + 1, tl, pf);
 
     // this thread's first cursor product of tile J: its load completes behind
     // the bitmap-tier products
@@ -366,29 +425,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     if (k0 < pe) hp = cprod[k0];
 
     // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
-    // Two passes of I_PER_THREAD/2 entries: the LDS lookups of a pass are
+    // Two passes of half the entries: the LDS lookups of a pass are
     // issued together, then its segments are walked.
 #pragma unroll
-    for (int h = 0; h < I_PER_THREAD; h += I_PER_THREAD / 2) {
-      int y0[I_PER_THREAD / 2], y1[I_PER_THREAD / 2];
+    for (int h = 0; h < Z::IPT; h += Z::IPT / 2) {
+      int y0[Z::IPT / 2], y1[Z::IPT / 2];
 #pragma unroll
-      for (int q = 0; q < I_PER_THREAD / 2; ++q) {
+      for (int q = 0; q < Z::IPT / 2; ++q) {
         int rf = e_rf[h + q];
         asm volatile("" : "+v"(rf));  // no hoisting of per-entry values out of the J loop
-        y0[q] = 0; y1[q] = 0;
-        if (rf >= 0) {
-          const int f = rf & 0xfff;  // bitmap index (bits 0..11)
-          const int w = f >> 6, bit = f & 63;
-          const uint64_t wj = bJ[w];
-          if ((wj >> bit) & 1ull) {
-            const int rk = wpJ[w] + __popcll(wj & ((1ull << bit) - 1ull));
-            y0[q] = cJ[rk];
-            y1[q] = cJ[rk + 1];
-          }
-        }
+        const int f = rf >= 0 ? (rf & 0xfff) : 0;  // bitmap index (bits 0..11)
+        const int s0 = cJ[f], s1 = cJ[f + 1];
+        y0[q] = rf >= 0 ? s0 : 0;
+        y1[q] = rf >= 0 ? s1 : 0;
       }
 #pragma unroll
-      for (int q = 0; q < I_PER_THREAD / 2; ++q) {
+      for (int q = 0; q < Z::IPT / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
         my_visits += 1u + (((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u);
         const int ib = (e_rf[h + q] >> 12) & 0x3fff;
@@ -418,9 +470,80 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         atomicAdd(&tile[q.x], q.y);
       }
     }
-    __syncthreads();
+
+    // ---- 2b. dense head on the matrix cores ------------------------------------
+    // Wave w owns the 16x16 blocks (rows 16*(w%8).., columns 32*(w/8) + 16*blk..)
+    // of the tile, blk = 0, 1; in the MFMA output layout lane l holds column
+    // l%16 and rows 4*(l/16) + v, v = 0..3, i.e. cell m = 4*blk + v.  The
+    // operands: lane l reads row l%16 of the block, chunk 4*s + l/16 of the
+    // head record (s = limb for int8, 32-feature step for BF16).
+    const int wv = tl >> 6, ln = tl & 63;
+    int32_t dv[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) dv[m] = 0;
+    if constexpr (DM != DENSE_OFF) {
+      const int r15 = ln & 15, g = ln >> 4;
+      const int rowA = 16 * (wv & 7) + r15;
+      v4i av[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[q] = __builtin_bit_cast(v4i, dA[dslot(rowA, 4 * q + g)]);
+#pragma unroll
+      for (int blk = 0; blk < 2; ++blk) {
+        const int rowB = 16 * (2 * (wv >> 3) + blk) + r15;
+        v4i bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[q] = __builtin_bit_cast(v4i, dB[dslot(rowB, 4 * q + g)]);
+        if constexpr (DM == DENSE_I8) {
+          // limb products grouped by weight 2^(8*(6-w)); groups w <= 4 are
+          // kept (the dropped w = 5, 6 are inside eps_dense)
+          const v4i z = {0, 0, 0, 0};
+          v4i c0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[0], z, 0, 0, 0);
+          v4i c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[1], z, 0, 0, 0);
+          v4i c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[2], z, 0, 0, 0);
+          v4i c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[3], z, 0, 0, 0);
+          v4i c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[3], z, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[0], c1, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[1], c2, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[2], c3, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[2], c4, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[0], c2, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[1], c3, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[1], c4, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[0], c3, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int32_t hi = (c0[v] << 16) + (c1[v] << 8) + c2[v];  // weight 2^32
+            const int32_t lo = (c3[v] << 8) + c4[v];                   // weight 2^16
+            dv[4 * blk + v] = (hi << a.dsh_hi) + (lo >> a.dsh_lo);
+          }
+        } else {
+          v4f c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, av[q]),
+                                                        __builtin_bit_cast(v8bf, bv[q]), c, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) dv[4 * blk + v] = (int)(c[v] * a.qscale);
+        }
+      }
+    }
+    lds_barrier();
+
+    // this thread's eight cells in the MFMA output layout
+    const int il_b = 16 * (wv & 7) + 4 * (ln >> 4);
+    const int jl_b = 32 * (wv >> 3) + (ln & 15);
+
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
     if (a.head_mode) {
+      unsigned int my_head = 0;
+      if constexpr (DM != DENSE_OFF) {  // the filter needs the dense part in the tile
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          tile[(il_b + (m & 3)) * TSTRIDE + jl_b + 16 * (m >> 2)] += dv[m];
+          dv[m] = 0;
+        }
+        lds_barrier();
+      }
 #pragma unroll 1
       for (int c = tl; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
         const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
@@ -429,9 +552,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int32_t part = *cell;
         // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
         const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jl];
-        if (bound + cell_error_bound(lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl], normI[il], normJ[jl],
-                                     a.qinv, 0.0) < t_band)
-          continue;
+        double eps0 = cell_error_bound(lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl], normI[il],
+                                       normJ[jl], a.qinv, 0.0);
+        if constexpr (DM == DENSE_I8) eps0 += a.eps_dense;
+        if constexpr (DM == DENSE_BF16) eps0 += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
+        if (bound + eps0 < t_band) continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
         const float h = head_dot(a.I.col + ai, a.I.val + ai, a.I.rp[ri + 1] - ai, a.J.col + aj,
@@ -439,48 +564,119 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         *cell = part + __float2int_rn(h * a.qscale);
         ++my_head;
       }
-      __syncthreads();
+      if (my_head) atomicAdd(&acc_head, (unsigned long long)my_head);
+      lds_barrier();
     }
 
     // ---- 4. threshold, compaction, reset -------------------------------------
-    // Pass 1 reads this thread's cells together and marks the ones that can
-    // reach t; pass 2 (rolled) settles candidates and clears every cell.
-    // cell c = tl + m*SCAN_THREADS: column tl % TILE_ROWS is fixed per thread,
-    // row advances by SCAN_THREADS / TILE_ROWS; a cell is valid when its row
-    // is below both the band height and (self-joins) the diagonal limit
-    constexpr int CELLS_PER_THREAD = BAND_ROWS * TILE_ROWS / SCAN_THREADS;
-    constexpr int ROW_STEP = SCAN_THREADS / TILE_ROWS;
-    static_assert(SCAN_THREADS % TILE_ROWS == 0, "fixed column per thread");
-    const int jl_c = tl % TILE_ROWS, il_0 = tl / TILE_ROWS;
-    int row_lim = jl_c < nJ ? nI : 0;
-    if (a.self) {  // i0 + il < j0 + jl  <=>  il < (j0 - i0) + jl
-      const int64_t dlim = j0 - i0 + jl_c;
-      row_lim = (int)(dlim < row_lim ? (dlim < 0 ? 0 : dlim) : row_lim);
+    // Pass 1 reads and clears this thread's eight cells (+ their dense parts)
+    // and appends the ones that can reach t to a candidate list in LDS (the
+    // tile's entry block, dead by now).  Pass 2 settles the candidates with one
+    // thread each.  Valid cells: row below the band height, column below the
+    // tile height, and for self-joins i0 + il < j0 + jl.  Candidates beyond the
+    // list capacity stay in their cells and are settled by their own thread.
+    int2* const cand = eJ;
+    const int64_t dj = j0 - i0;
+    if (tl == 0) {  // valid cells of the tile pair (closed form of the per-column limits)
+      int64_t c;
+      if (!a.self) {
+        c = (int64_t)nI * nJ;
+      } else {  // sum over jl < nJ of clamp(dj + jl, 0, nI)
+        auto G = [nI](int64_t y) -> int64_t {  // sum of clamp(x, 0, nI) over 0 <= x < y
+          if (y <= 0) return 0;
+          if (y <= (int64_t)nI + 1) return y * (y - 1) / 2;
+          return (int64_t)nI * (nI + 1) / 2 + (y - nI - 1) * nI;
+        };
+        c = G(dj + nJ) - G(dj);
+      }
+      acc_cells += (unsigned long long)c;
     }
-    unsigned cand = 0;
+    // Cells past the band or tile height receive no products and a zero head,
+    // so with q_lo > 0 only the self-join diagonal needs an explicit validity
+    // test: off the diagonal one max over the eight cells rejects them all.
+    const bool plain = q_lo > 0 && (!a.self || dj >= BAND_ROWS);
+    int32_t qmax = INT32_MIN;
 #pragma unroll
-    for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-      const int il = il_0 + m * ROW_STEP;
-      const int32_t q = tile[il * TSTRIDE + jl_c];
-      const bool valid = il < row_lim;
-      my_cells += valid ? 1u : 0u;
-      cand |= (valid && q >= q_lo) ? (1u << m) : 0u;
-    }
-#pragma unroll 1
-    for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-      const int c = tl + m * SCAN_THREADS;
-      const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
-      int32_t* cell = &tile[il * TSTRIDE + jl];
-      const bool is_cand = (cand >> m) & 1u;
-      if (!__ballot(is_cand)) { *cell = 0; continue; }
-      const int32_t q = *cell;
+    for (int m = 0; m < 8; ++m) {
+      int32_t* cell = &tile[(il_b + (m & 3)) * TSTRIDE + jl_b + 16 * (m >> 2)];
+      dv[m] += *cell;  // full fixed-point score of the cell
       *cell = 0;
+      qmax = dv[m] > qmax ? dv[m] : qmax;
+    }
+    unsigned cmask = 0;
+    if (__ballot(!plain || qmax >= q_lo)) {
+      // valid rows of this thread's two blocks: v < lim[blk] (v = m % 4)
+      int lim[2];
+#pragma unroll
+      for (int blk = 0; blk < 2; ++blk) {
+        const int jl = jl_b + 16 * blk;
+        int l = jl < nJ ? nI - il_b : 0;
+        if (a.self) {  // i0 + il < j0 + jl  <=>  il - il_b < dj + jl - il_b
+          const int64_t dl = dj + jl - il_b;
+          l = dl < l ? (int)(dl < 0 ? 0 : dl) : l;
+        }
+        lim[blk] = l < 0 ? 0 : (l > 4 ? 4 : l);
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        cmask |= ((m & 3) < lim[m >> 2] && dv[m] >= q_lo) ? (1u << m) : 0u;
+    }
+    unsigned ovf = 0;
+    if (__ballot(cmask != 0)) {  // rare: append this wave's candidates to the list
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const bool c = (cmask >> m) & 1u;
+        const unsigned long long mc = __ballot(c);
+        if (!mc) continue;
+        const int leader = __ffsll((long long)mc) - 1;
+        unsigned int base = 0;
+        if (ln == leader) base = atomicAdd(&cand_n, (unsigned)__popcll(mc));
+        const unsigned int pos = __shfl(base, leader) + (unsigned)lane_rank(mc);
+        const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
+        if (c && pos < (unsigned)Z::CJ) cand[pos] = make_int2((il << 8) | jl, dv[m]);
+        if (c && pos >= (unsigned)Z::CJ) {  // list full: the cell keeps its score
+          tile[il * TSTRIDE + jl] = dv[m];
+          ovf |= 1u << m;
+        }
+      }
+    }
+    lds_barrier();
+    const int n_list = (int)(cand_n < (unsigned)Z::CJ ? cand_n : (unsigned)Z::CJ);
+    const int n_ovf = (int)cand_n - n_list;
+    for (int k0c = 0; k0c < n_list + (n_ovf > 0 ? 8 * SCAN_THREADS : 0); k0c += SCAN_THREADS) {
+      // rounds over the list, then (only after an overflow) one round per own cell
+      const int kk = k0c + tl;
+      bool is_cand = false;
+      int il = 0, jl = 0;
+      int32_t q = 0;
+      if (k0c < n_list) {
+        if (kk < n_list) {
+          const int2 r = cand[kk];
+          il = r.x >> 8;
+          jl = r.x & 0xff;
+          q = r.y;
+          is_cand = true;
+        }
+      } else {
+        const int m = (k0c - n_list) / SCAN_THREADS;
+        il = il_b + (m & 3);
+        jl = jl_b + 16 * (m >> 2);
+        if ((ovf >> m) & 1u) {
+          int32_t* cell = &tile[il * TSTRIDE + jl];
+          q = *cell;
+          *cell = 0;
+          is_cand = true;
+        }
+      }
+      if (!__ballot(is_cand)) continue;
       bool emit = false, to_band = false;
       double sd = 0.0;
       if (is_cand) {
         const int k = lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl];
         const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jl] : 0.0;
-        const double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
+        double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
+        if constexpr (DM == DENSE_I8) eps += a.eps_dense;
+        if constexpr (DM == DENSE_BF16) eps += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
         sd = (double)q * a.qinv;
         if (sd - eps >= t_band) {
           // certain hit; its score is final only if the bound meets the relative tolerance
@@ -518,38 +714,40 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if (to_band && bslot < a.band_cap)
         a.band[bslot] = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
     }
-    __syncthreads();
+    lds_barrier();
+    if (tl == 0) cand_n = 0;  // read by every thread before the barrier above
     // flush the LDS output buffer when half full, and after the last tile
     const unsigned int nbuf = obuf_n;
     if (nbuf >= OBUF_CAP / 2 || (J + 1 == J_end && nbuf > 0)) {
       const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
       if (tl == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
-      __syncthreads();
+      lds_barrier();
       const unsigned long long ob = obuf_base;
       for (unsigned int k = tl; k < nf; k += SCAN_THREADS)
         if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
-      __syncthreads();
+      lds_barrier();
       if (tl == 0) obuf_n = 0;
-      __syncthreads();
+      lds_barrier();
     }
   }
   __syncthreads();
   }  // band queue
 
-  atomicAdd(&acc_products, (unsigned long long)my_products + my_cursor);
-  atomicAdd(&acc_head, (unsigned long long)my_head);
-  atomicAdd(&acc_cells, (unsigned long long)my_cells);
+  atomicAdd(&acc_products, (unsigned long long)my_products);
   atomicAdd(&acc_visits, (unsigned long long)my_visits);
-  atomicAdd(&acc_cursor, (unsigned long long)my_cursor);
   __syncthreads();
   if (tid0 == 0) {
-    atomicAdd(&a.stats[0], acc_products);
+    atomicAdd(&a.stats[0], acc_products + acc_cursor);
     atomicAdd(&a.stats[1], acc_head);
     atomicAdd(&a.stats[2], acc_cells);
     atomicAdd(&a.stats[3], acc_visits);
     atomicAdd(&a.stats[4], acc_cursor);
   }
 }
+
+template __global__ void k_band_scan<DENSE_OFF>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16>(ScanArgs);
 
 // FP64 rescore of the band: exact products summed in ascending feature id,
 // the same order as the FP64 oracle, so the decision s >= t is settled exactly.

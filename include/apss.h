@@ -16,12 +16,17 @@
  *     (the reference never normalises; inputs are expected L2-normalised);
  *   - the result is every pair i < j with score >= t (self-join), or every
  *     (query, corpus) pair with score >= t (index queries);
- *   - feature ids < p form the head (forward-list dot product, or BF16 MFMA
- *     tiles in APSS_PREC_BF16_HEAD mode), ids >= p the tail (inverted lists);
- *     the emitted pair set does not depend on p;
+ *   - feature ids < p form the head (forward-list dot product), ids >= p the
+ *     tail (inverted lists; the most frequent tail features go to a dense
+ *     MFMA head); the emitted pair set does not depend on p;
  *   - the pair set equals the exact FP64 result (products exact in double,
  *     summed in ascending feature id): every pair whose fast-path score lies
  *     within its rigorous rounding bound of t is rescored in FP64 on the GPU;
+ *   - APSS_PREC_FP32: every emitted score is within 1e-6 relative of the FP64
+ *     score.  APSS_PREC_BF16_HEAD: the pair set is still exact; an emitted
+ *     score is within (2^-7 + 2^-16 + 2*gamma_128) * |x_H| |y_H| + 2^-S +
+ *     (FP32 tail bound) of the FP64 score, |x_H| the norm of the row's
+ *     features in the BF16 head (at most 7.9e-3 for unit rows);
  *   - errors are returned as negative APSS_E_* codes; nothing calls exit().
  */
 #ifndef APSS_H
@@ -83,7 +88,12 @@ typedef struct {
   int32_t dense_features;  /* cap of the bitmap tier (0 = automatic) */
   int64_t mem_budget;      /* device bytes a call may allocate (0 = no limit) */
   void* stream;            /* hipStream_t to run on (NULL = library-owned stream) */
-  int64_t reserved_[4];
+  int32_t mfma_head;       /* dense MFMA head of the most frequent tail features:
+                              0 = on (64 features as exact int8 limbs in FP32 mode,
+                              128 BF16 features in APSS_PREC_BF16_HEAD mode),
+                              -1 = off (every feature on the sparse path) */
+  int32_t reserved32_;
+  int64_t reserved_[3];
 } apss_opts;
 
 typedef struct {
@@ -98,6 +108,7 @@ typedef struct {
   int64_t head_pairs;      /* pairs that ran the head forward-list dot */
   int64_t list_visits;     /* inverted-list entries visited by tile pairs (both sides) */
   int64_t cursor_products; /* products of the short-list (cursor) tier */
+  int64_t mfma_features;   /* features in the dense MFMA head (0: none) */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   void* internal_;

@@ -32,3 +32,15 @@ def check_pairs(got, exp, rel=1e-6):
     err = np.abs(g["score"].astype(np.float64) - exp["score"]) / np.maximum(np.abs(exp["score"]), 1e-30)
     assert err.max() <= rel, f"max relative score error {err.max():.3e} > {rel}"
     return float(err.max())
+
+
+def check_pairs_abs(got, exp, atol):
+    """Exact pair-set equality and scores within `atol` absolute (BF16 head)."""
+    g = apss.sort_pairs(got)
+    assert len(g) == len(exp), f"pair count {len(g)} != oracle {len(exp)}"
+    if len(g) == 0:
+        return 0.0
+    assert np.array_equal(g["i"], exp["i"]) and np.array_equal(g["j"], exp["j"]), "pair sets differ"
+    err = np.abs(g["score"].astype(np.float64) - exp["score"])
+    assert err.max() <= atol, f"max absolute score error {err.max():.3e} > {atol}"
+    return float(err.max())

@@ -9,7 +9,7 @@ import pytest
 
 import apss
 import oracle
-from helpers import check_pairs, subset, tiny_matrix
+from helpers import check_pairs, check_pairs_abs, subset, tiny_matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -147,3 +147,49 @@ def test_dense_output_overflows_buffers():
     rp, col, val = subset(rp, col, val, 1200)
     got, exp, info, err = run_and_check(rp, col, val, d, 0.01, 0)
     assert len(exp) > 100_000
+
+
+# ---- dense MFMA head ---------------------------------------------------------
+
+@pytest.mark.parametrize("mfma_head", [0, -1])
+@pytest.mark.parametrize("p", [0, 64])
+def test_dense_head_on_and_off(mfma_head, p):
+    rp, col, val, d = apss.workload("large", n=5000)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.4, p, mfma_head=mfma_head)
+    assert info["mfma_features"] == (64 if mfma_head == 0 else 0)
+
+
+def test_dense_head_signed_and_unnormalised_values():
+    # random signs exercise the balanced limb digits; a row scale of up to 3.7
+    # moves the fixed-point exponents away from the unit-norm case
+    rp, col, val, d = apss.workload("mixed", n=4000)
+    rng = np.random.default_rng(7)
+    sign = np.where(rng.random(len(val)) < 0.3, -1.0, 1.0).astype(np.float32)
+    scale = np.repeat(rng.uniform(0.2, 3.7, len(rp) - 1), np.diff(rp)).astype(np.float32)
+    v2 = (val * sign * scale).astype(np.float32)
+    for t in (0.5, 2.0):
+        got, info = apss.run(rp, col, v2, d, t, 0)
+        check_pairs(got, oracle.apss_fp64(rp, col, v2, d, t))
+        assert info["mfma_features"] == 64
+
+
+@pytest.mark.parametrize("t", [0.5, 0.8])
+def test_bf16_head_pair_set_exact(t):
+    # the BF16 head keeps the pair set exact (band rescored in FP64); scores
+    # of unit rows carry at most (2^-7 + 2^-16 + 2 gamma_128) + FP32 tail error
+    rp, col, val, d = apss.workload("mixed", n=6000)
+    got, info = apss.run(rp, col, val, d, t, 0, precision=apss.APSS_PREC_BF16_HEAD)
+    exp = oracle.apss_fp64(rp, col, val, d, t)
+    err = check_pairs_abs(got, exp, 7.9e-3)
+    assert info["mfma_features"] == 128
+    assert info["band"] > 0  # the BF16 bound sends near-threshold cells to the FP64 rescore
+    assert err > 0.0
+
+
+def test_bf16_head_with_split_and_rescore_all():
+    rp, col, val, d = apss.workload("large", n=3000)
+    exp = oracle.apss_fp64(rp, col, val, d, 0.45)
+    got, _ = apss.run(rp, col, val, d, 0.45, 32, precision=apss.APSS_PREC_BF16_HEAD)
+    check_pairs_abs(got, exp, 7.9e-3)
+    got, _ = apss.run(rp, col, val, d, 0.45, 0, precision=apss.APSS_PREC_BF16_HEAD, rescore_all=1)
+    check_pairs(got, exp)  # every score from the FP64 rescore
