@@ -46,6 +46,7 @@ class Result(ctypes.Structure):
                 ("p_used", c_int32), ("cells", c_int64), ("band", c_int64),
                 ("products", c_int64), ("head_pairs", c_int64), ("list_visits", c_int64),
                 ("cursor_products", c_int64), ("mfma_features", c_int64),
+                ("dense_rows", c_int64), ("tile_pairs", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
                 ("internal_", c_void_p)]
@@ -158,6 +159,23 @@ def workload(name, n=None, seed_offset=0):
                     seed=SEED_BASE + w["index"] + seed_offset) + (w["d"],)
 
 
+def batched_instances(count=256, n_lo=500, n_hi=50_000, scale=1.0):
+    """The batched workload: `count` independent instances, n log-uniform in
+    [n_lo, n_hi] (times `scale`), mean nnz/row uniform in [20, 200], d = 1e5,
+    Zipf(1.1) ids, uniform(0,1] values.  Instance k uses seed base + 2 + k
+    (instance parameters from a generator seeded with base + 2)."""
+    w = WORKLOADS["batched"]
+    rng = np.random.default_rng(SEED_BASE + w["index"])
+    ns = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), size=count))
+    means = rng.uniform(20.0, 200.0, size=count)
+    out = []
+    for k in range(count):
+        n = max(1, int(round(ns[k] * scale)))
+        out.append(generate(n, w["d"], float(means[k]), w["kmin"], w["kmax"], w["zipf"], w["values"],
+                            seed=SEED_BASE + w["index"] + k) + (w["d"],))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # calls
 
@@ -199,6 +217,37 @@ def run(rp, col, val, d, t, p=0, **opts):
     finally:
         L.apss_result_free(ctypes.byref(r))
     return pairs, info
+
+
+EMIT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+
+
+def run_cb(rp, col, val, d, t, p, emit, **opts):
+    """Streaming self-join through apss_run_cb: emit(pairs) receives each chunk
+    as a PAIR_DTYPE view that is valid only during the call.  Returns info."""
+    L = lib()
+    L.apss_run_cb.argtypes = [POINTER(CSR), c_double, c_int32, POINTER(Opts), EMIT_FN, c_void_p,
+                              POINTER(Result)]
+    L.apss_run_cb.restype = c_int32
+    X = make_csr(rp, col, val, d)
+    o = make_opts(**opts)
+    r = Result()
+    errors = []
+
+    def _emit(ptr, count, _user):
+        try:
+            buf = (ctypes.c_char * (count * 12)).from_address(ptr)
+            emit(np.frombuffer(buf, dtype=PAIR_DTYPE, count=count))
+        except Exception as e:  # noqa: BLE001 -- re-raised after the call
+            errors.append(e)
+
+    cb = EMIT_FN(_emit)
+    rc = L.apss_run_cb(ctypes.byref(X), float(t), int(p), ctypes.byref(o), cb, None, ctypes.byref(r))
+    if rc != APSS_OK:
+        raise ApssError(rc, L.apss_last_error().decode())
+    if errors:
+        raise errors[0]
+    return {f: getattr(r, f) for f, _ in Result._fields_ if f not in ("pairs", "internal_")}
 
 
 def sort_pairs(pairs):
@@ -402,12 +451,17 @@ class Server:
         return tk
 
     def wait(self, ticket):
+        pairs, _, lat = self.wait_info(ticket)
+        return pairs, lat
+
+    def wait_info(self, ticket):
+        """(pairs, counters of the launch that served the request, latency ms)."""
         r = Result()
         lat = c_double()
         rc = self.L.apss_server_wait(self.h, int(ticket), ctypes.byref(r), ctypes.byref(lat))
         _check(rc)
         pairs, info = _take_result(r)
-        return pairs, lat.value
+        return pairs, info, lat.value
 
     def cancel(self, ticket):
         return self.L.apss_server_cancel(self.h, int(ticket))

@@ -33,7 +33,8 @@ constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
 constexpr int CAP_I = 8192;                  // bitmap-tier entries of a band
 constexpr int CAP_J = 4096;                  // bitmap-tier entries of a tile
 constexpr int CAP_S = 8192;                  // cursor-tier records of a band (band cut)
-constexpr int S_LIST_MAX = 8192;             // longest list of the cursor tier
+constexpr int S_LIST_MAX = 8192;             // longest list sorted in LDS in one piece
+constexpr int MERGE_TILE = 2048;             // outputs per workgroup of a merge pass
 constexpr int CH_TILES = 1024;               // J tiles per cursor-bucket chunk
 
 // Dense MFMA head: the most frequent tail features, stored per row as a
@@ -152,25 +153,41 @@ struct ScanArgs {
   unsigned long long* band_cnt;
   int64_t band_cap;
   int32_t rescore_all;
-  int32_t band_first;        // first band (tile row of I) of this launch
-  int32_t band_count;
-  int32_t pad2_;
-  unsigned long long* stats; // [0] products [1] head pairs [2] cells [3] list visits [4] cursor products
-  const int32_t* band_jend;  // per band: one past its last partner tile (NULL: all)
+  int32_t n_items;           // work items of this launch
+  const int4* items;         // {band I, first J tile, one past the last J tile, unused}
+  unsigned long long* stats; // [0] products [1] head pairs [2] cells [3] list visits
+                             // [4] cursor products [5] dense records fetched [6] tile pairs
   const double* band_t;      // per band: threshold (NULL: t)
   double eps_max;            // largest cell error bound (for per-band fast rejects)
-  int4* scs;                 // per cursor record: {position, end, J tile at position
-                             // (INT32_MAX: exhausted), unused}
+  int4* scs;                 // per cursor record of the current band: {position, end,
+                             // J tile at position (INT32_MAX: exhausted), unused};
+                             // one region of scs_cap records per workgroup
+  int64_t scs_cap;
   // cursor-tier products, bucketed by J tile for one chunk of CH_TILES tiles;
   // one region of cprod_cap records per workgroup of the persistent grid
   int2* cprod;               // {cell = il * TSTRIDE + jl, fixed-point product}
   int64_t cprod_cap;
-  unsigned long long* band_next;  // band queue of the persistent grid
+  unsigned long long* item_next;  // work-item queue of the persistent grid
   // dense head: int8 limb groups combine as hi << dsh_hi + (lo >> dsh_lo);
   // eps_dense bounds the int8 head's error, c_bf * |x_D||y_D| the BF16 head's
   int32_t dsh_hi, dsh_lo;
   double eps_dense;
   double c_bf;
+};
+
+// A piece of a cursor list sorted in LDS, and one merge-pass work unit: the
+// sorted runs [a, a+la) and [a+la, a+la+lb) merge into the same positions of
+// the other buffer; this unit writes outputs [k*MERGE_TILE, (k+1)*MERGE_TILE).
+struct SortChunk {
+  int64_t start;
+  int32_t len;
+  int32_t pad_;
+};
+struct MergeTask {
+  int64_t a;
+  int32_t la, lb;
+  int32_t k;
+  int32_t pad_;
 };
 
 // Arguments of the per-tile bitmap-tier build kernels.

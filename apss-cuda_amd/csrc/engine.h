@@ -1,5 +1,6 @@
 // Host-side engine of libapss.so: device memory, index build, scan launches.
 #pragma once
+#include <map>
 #include <memory>
 
 #include "apss_internal.h"
@@ -17,7 +18,8 @@ __global__ void k_col_hist(int64_t nnz, const int32_t* col, int32_t p, int64_t c
 __global__ void k_cursor_scatter(int64_t n, const int64_t* rp, const int32_t* col, const float* val,
                                  const int32_t* fmap, unsigned long long* cur, int2* sent);
 __global__ void k_sort_small(const int32_t* lists, int32_t nl, const int64_t* sptr, int2* sent);
-__global__ void k_sort_large(const int32_t* lists, const int64_t* sptr, int2* sent);
+__global__ void k_sort_chunks(const SortChunk* chunks, int2* sent);
+__global__ void k_merge_runs(const MergeTask* tasks, const int2* src, int2* dst);
 __global__ void k_row_counts(int64_t n, const int64_t* rp, const int32_t* col, int32_t p,
                              const int32_t* fmap, int32_t* dcnt, int32_t* scnt);
 __global__ void k_tile_count(BuildSide s);
@@ -41,11 +43,32 @@ __global__ void k_rescore(const unsigned long long* band, int64_t nband, const i
 
 // ---- device memory --------------------------------------------------------
 
+// Device blocks kept for reuse by one serving slot: blocks freed by an arena
+// that draws on the pool go back to it instead of to hipFree, so the request
+// loop stops calling hipMalloc / hipFree (which synchronise the device) once
+// the pool has grown to its working set.  Not thread-safe: one pool per slot.
+class DevicePool {
+ public:
+  DevicePool() = default;
+  ~DevicePool();
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
+  // A free block of at least `bytes` (and at most twice that), or nullptr.
+  void* take(size_t bytes, size_t* got);
+  void give(void* p, size_t bytes) { free_.emplace(bytes, p); }
+  int64_t held() const;
+
+ private:
+  std::multimap<size_t, void*> free_;
+};
+
 // Owns every device allocation of one call (or one index); enforces a byte
-// budget and frees everything on destruction.
+// budget and frees everything on destruction (or returns it to a pool).
 class Arena {
  public:
-  Arena(int device, int64_t budget) : device_(device), budget_(budget) {}
+  Arena(int device, int64_t budget, DevicePool* pool = nullptr)
+      : device_(device), budget_(budget), pool_(pool) {}
+  int device() const { return device_; }
   ~Arena();
   Arena(const Arena&) = delete;
   Arena& operator=(const Arena&) = delete;
@@ -60,10 +83,17 @@ class Arena {
 
  private:
   void* raw(size_t bytes);
+  void drop(void* p, size_t bytes);
   int device_;
   int64_t budget_;
+  DevicePool* pool_;
   int64_t used_ = 0, peak_ = 0;
-  std::vector<std::pair<void*, size_t>> blocks_;
+  struct Blk {
+    void* p;
+    size_t size;  // bytes of the block (a pooled block may exceed the request)
+    size_t req;   // bytes counted against the budget
+  };
+  std::vector<Blk> blocks_;
 };
 
 // ---- index pieces -----------------------------------------------------------
@@ -139,15 +169,17 @@ struct RunConfig {
   int32_t rescore_all = 0;
   int32_t kd_cap = 0;
   int64_t i_off = 0, j_off = 0;
-  const int32_t* band_jend = nullptr;  // device, per band (batched runs)
-  const double* band_t = nullptr;      // device, per band (batched runs)
+  std::vector<int32_t> band_jend;      // per band: one past its last partner tile (empty: all)
+  const double* band_t = nullptr;      // device, per band (batched runs, packed queries)
   double t_min = 0.0;                  // smallest per-band threshold
+  int64_t out_cap_hint = 0;            // initial output capacity in pairs (0: from free memory)
 };
 
 struct ScanOutput {
   apss_pair* pairs = nullptr;   // device, owned by the arena
   int64_t count = 0;
   int64_t band = 0, products = 0, head_pairs = 0, cells = 0, visits = 0, cursor = 0;
+  int64_t dense_rows = 0, tile_pairs = 0;
   int32_t n_dense = 0;
   double ms_scan = 0, ms_rescore = 0;
 };

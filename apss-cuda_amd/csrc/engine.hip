@@ -11,6 +11,7 @@
 // counters.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -21,8 +22,32 @@ namespace apss {
 // ---------------------------------------------------------------------------
 // Arena
 
+DevicePool::~DevicePool() {
+  for (auto& b : free_) (void)hipFree(b.second);
+}
+
+void* DevicePool::take(size_t bytes, size_t* got) {
+  auto it = free_.lower_bound(bytes);
+  if (it == free_.end() || it->first > 2 * bytes + (1 << 20)) return nullptr;
+  void* p = it->second;
+  *got = it->first;
+  free_.erase(it);
+  return p;
+}
+
+int64_t DevicePool::held() const {
+  int64_t s = 0;
+  for (auto& b : free_) s += (int64_t)b.first;
+  return s;
+}
+
 Arena::~Arena() {
-  for (auto& b : blocks_) (void)hipFree(b.first);
+  for (auto& b : blocks_) drop(b.p, b.size);
+}
+
+void Arena::drop(void* p, size_t bytes) {
+  if (pool_) pool_->give(p, bytes);
+  else (void)hipFree(p);
 }
 
 void* Arena::raw(size_t bytes) {
@@ -31,11 +56,16 @@ void* Arena::raw(size_t bytes) {
     throw Error{APSS_E_NOMEM, "device memory budget exceeded (" + std::to_string(used_ + bytes) +
                                   " > " + std::to_string(budget_) + " bytes)"};
   void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess || !p) {
-    (void)hipGetLastError();
-    throw Error{APSS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+  size_t got = bytes;
+  if (pool_) p = pool_->take(bytes, &got);
+  if (!p) {
+    got = bytes;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      throw Error{APSS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
   }
-  blocks_.push_back({p, bytes});
+  blocks_.push_back({p, got, bytes});
   used_ += (int64_t)bytes;
   peak_ = std::max(peak_, used_);
   return p;
@@ -43,9 +73,9 @@ void* Arena::raw(size_t bytes) {
 
 void Arena::release(void* p) {
   for (size_t k = 0; k < blocks_.size(); ++k)
-    if (blocks_[k].first == p) {
-      (void)hipFree(p);
-      used_ -= (int64_t)blocks_[k].second;
+    if (blocks_[k].p == p) {
+      drop(p, blocks_[k].size);
+      used_ -= (int64_t)blocks_[k].req;
       blocks_.erase(blocks_.begin() + (long)k);
       return;
     }
@@ -53,8 +83,8 @@ void Arena::release(void* p) {
 
 void* Arena::detach(void* p) {
   for (size_t k = 0; k < blocks_.size(); ++k)
-    if (blocks_[k].first == p) {
-      used_ -= (int64_t)blocks_[k].second;
+    if (blocks_[k].p == p) {
+      used_ -= (int64_t)blocks_[k].req;
       blocks_.erase(blocks_.begin() + (long)k);
       return p;
     }
@@ -83,6 +113,25 @@ double Timer::stop_ms() {
 
 // ---------------------------------------------------------------------------
 // helpers
+
+// APSS_TRACE=1 prints the host wall time of each phase of a job to stderr.
+static bool trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("APSS_TRACE");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+struct PhaseTrace {
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char* what) {
+    if (!trace_on()) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[apss] %-28s %9.2f ms\n", what,
+                 std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+};
 
 static void fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
   if (n <= 0) return;
@@ -235,8 +284,9 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 
   const int32_t need = self ? 2 : 1;
   // bitmap tier: the lists present in a sizeable share of the 64-row tiles,
-  // plus every list too long for the cursor tier; the most frequent of them
-  // form the dense MFMA head when there is one
+  // plus every list longer than one LDS sort piece (the cursor tier would walk
+  // them entry by entry), most frequent first, up to the tier's capacity; the
+  // most frequent of them form the dense MFMA head when there is one
   const int64_t T = std::max<int64_t>(1, (s.n + TILE_ROWS - 1) / TILE_ROWS);
   const int64_t lmin = std::max<int64_t>(need, T / 8);
   std::vector<int32_t> cand;
@@ -250,12 +300,6 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   cand.erase(cand.begin(), cand.begin() + n_dense);
   const int32_t kdm = kd_max_for(dm);
   int32_t cap = kd_cap > 0 ? std::min(kd_cap, kdm) : kdm;
-  int64_t n_forced = 0;
-  for (int32_t c : cand) n_forced += h[c] > S_LIST_MAX;
-  if (n_forced > cap)
-    throw Error{APSS_E_UNSUPPORTED, std::to_string(n_forced) + " features have lists longer than " +
-                                        std::to_string(S_LIST_MAX) + " but the bitmap tier holds " +
-                                        std::to_string(cap)};
   if ((int64_t)cand.size() > cap) cand.resize(cap);
   tr.dm = dm;
   tr.n_dense = n_dense;
@@ -272,12 +316,17 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
   for (size_t k = 0; k < dense.size(); ++k) fm[dense[k]] = F_DENSE0 - (int32_t)k;
   std::vector<int64_t> sp((size_t)d + 1, 0);
-  std::vector<int32_t> small_lists, large_lists;
+  std::vector<int32_t> small_lists;
+  std::vector<SortChunk> chunks;  // lists of 65.. entries, cut in LDS-sized pieces
+  int64_t max_long = 0;           // longest list cut in more than one piece
   for (int64_t c = 0; c < d; ++c) {
     int64_t L = fm[c] == F_CURSOR ? h[c] : 0;
     sp[c + 1] = sp[c] + L;
     if (L >= 2 && L <= 64) small_lists.push_back((int32_t)c);
-    else if (L > 64) large_lists.push_back((int32_t)c);
+    else if (L > 64)
+      for (int64_t s0 = 0; s0 < L; s0 += S_LIST_MAX)
+        chunks.push_back(SortChunk{sp[c] + s0, (int32_t)std::min<int64_t>(S_LIST_MAX, L - s0), 0});
+    if (L > S_LIST_MAX) max_long = std::max(max_long, L);
   }
   tr.n_cursor_entries = sp[d];
   if (tr.n_cursor_entries >= ((int64_t)1 << 31))
@@ -288,13 +337,14 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   APSS_HIP(hipMemcpyAsync(tr.fmap, fm.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
   APSS_HIP(hipMemcpyAsync(tr.sptr, sp.data(), sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
   if (tr.n_cursor_entries > 0) {
+    // scatter (list order within a feature is arbitrary), then sort each list by row
     unsigned long long* cur = ar.alloc<unsigned long long>(d + 1);
     hipLaunchKernelGGL(k_copy_i64, dim3(blocks_for(d + 1, 256, 4096)), dim3(256), 0, st,
                        (int64_t*)cur, tr.sptr, d + 1);
     hipLaunchKernelGGL(k_cursor_scatter, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n,
                        s.rp, s.col, s.val, tr.fmap, cur, tr.sent);
     APSS_HIP(hipGetLastError());
-    int32_t* lists = ar.alloc<int32_t>((int64_t)(small_lists.size() + large_lists.size()));
+    int32_t* lists = ar.alloc<int32_t>((int64_t)small_lists.size());
     if (!small_lists.empty()) {
       APSS_HIP(hipMemcpyAsync(lists, small_lists.data(), sizeof(int32_t) * small_lists.size(),
                               hipMemcpyHostToDevice, st));
@@ -302,15 +352,56 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
                          dim3(256), 0, st, lists, (int32_t)small_lists.size(), tr.sptr, tr.sent);
       APSS_HIP(hipGetLastError());
     }
-    if (!large_lists.empty()) {
-      int32_t* ll = lists + small_lists.size();
-      APSS_HIP(hipMemcpyAsync(ll, large_lists.data(), sizeof(int32_t) * large_lists.size(),
+    SortChunk* dch = ar.alloc<SortChunk>((int64_t)chunks.size());
+    if (!chunks.empty()) {
+      APSS_HIP(hipMemcpyAsync(dch, chunks.data(), sizeof(SortChunk) * chunks.size(),
                               hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_sort_large, dim3((unsigned)large_lists.size()), dim3(256), 0, st, ll,
-                         tr.sptr, tr.sent);
+      hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)chunks.size()), dim3(256), 0, st, dch, tr.sent);
       APSS_HIP(hipGetLastError());
     }
+    // lists longer than one piece: pairwise merges of sorted runs, doubling the
+    // run width each pass, ping-ponging between sent and a second buffer (every
+    // long list takes part in every pass, as a copy once it is one run)
+    if (max_long > 0) {
+      int2* other = ar.alloc<int2>(tr.n_cursor_entries);
+      int2* src = tr.sent;
+      int2* dst = other;
+      std::vector<MergeTask> tasks;
+      for (int64_t w = S_LIST_MAX; w < max_long; w *= 2) {
+        tasks.clear();
+        for (int64_t c = 0; c < d; ++c) {
+          const int64_t L = fm[c] == F_CURSOR ? h[c] : 0;
+          if (L <= S_LIST_MAX) continue;
+          for (int64_t s0 = 0; s0 < L; s0 += 2 * w) {
+            const int32_t la = (int32_t)std::min<int64_t>(w, L - s0);
+            const int32_t lb = (int32_t)std::max<int64_t>(0, std::min<int64_t>(w, L - s0 - w));
+            for (int32_t k = 0; (int64_t)k * MERGE_TILE < (int64_t)la + lb; ++k)
+              tasks.push_back(MergeTask{sp[c] + s0, la, lb, k, 0});
+          }
+        }
+        MergeTask* dt = ar.alloc<MergeTask>((int64_t)tasks.size());
+        APSS_HIP(hipMemcpyAsync(dt, tasks.data(), sizeof(MergeTask) * tasks.size(),
+                                hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_merge_runs, dim3((unsigned)tasks.size()), dim3(256), 0, st, dt, src, dst);
+        APSS_HIP(hipGetLastError());
+        APSS_HIP(hipStreamSynchronize(st));  // the task array is released next
+        ar.release(dt);
+        std::swap(src, dst);
+      }
+      // short lists were sorted in place in sent; the long ones end in src
+      if (src != tr.sent) {
+        for (int64_t c = 0; c < d; ++c) {
+          const int64_t L = fm[c] == F_CURSOR ? h[c] : 0;
+          if (L > S_LIST_MAX)
+            APSS_HIP(hipMemcpyAsync(tr.sent + sp[c], src + sp[c], sizeof(int2) * L,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+      }
+      APSS_HIP(hipStreamSynchronize(st));
+      ar.release(other);
+    }
     APSS_HIP(hipStreamSynchronize(st));
+    ar.release(dch);
     ar.release(lists);
     ar.release(cur);
   }
@@ -522,83 +613,162 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   const double qlo = std::floor((t_min - 2.0 * eps_max) * qscale);
   const int32_t q_lo = (int32_t)std::max(-2147483647.0, std::min(2147483647.0, qlo));
 
-  // pair-space size bounds the output capacity
   const double cells = rc.self ? 0.5 * (double)I.n * (double)(I.n - 1) : (double)I.n * (double)J.n;
-  size_t free_b = 0, total_b = 0;
-  APSS_HIP(hipMemGetInfo(&free_b, &total_b));
-  int64_t cap = (int64_t)std::min<double>(cells, (double)free_b * 0.25 / sizeof(apss_pair));
-  cap = std::max<int64_t>(cap, 1 << 16);
-  int64_t bcap = std::max<int64_t>(1 << 20, cap / 64);
-  unsigned long long* ctr = ar.alloc<unsigned long long>(8);
-  int4* scs = ar.alloc<int4>(bands.nS);
-  // persistent grid: one 1024-thread workgroup per CU pulls bands from a queue
-  // and owns one cursor-product region, large enough for every product one
-  // tile pair can receive (records x tile rows)
+  // counters: [0] output pairs [1] band pairs [2..8] kernel statistics [9] work-item queue
+  unsigned long long* ctr = ar.alloc<unsigned long long>(16);
+
+  // Work items: band b against a chunk of its partner tiles [J_first, J_end)
+  // (self-join: the first tile whose rows end after the band's first row).
+  // Chunks are sized so the items outnumber the workgroups several times
+  // over, which spreads a query batch of a few bands over the whole chip.
   int n_cu = 0, dev = 0;
   APSS_HIP(hipGetDevice(&dev));
   APSS_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::max(1, std::min(bands.T, std::max(n_cu, 1)));
+  n_cu = std::max(n_cu, 1);
+  std::vector<int32_t> jf((size_t)std::max(bands.T, 0)), je((size_t)std::max(bands.T, 0));
+  int64_t partner_tiles = 0;
+  for (int32_t b = 0; b < bands.T; ++b) {
+    int32_t f = 0;
+    if (rc.self)
+      f = (int32_t)(std::upper_bound(tiles.cuts.begin() + 1, tiles.cuts.end(), bands.cuts[b]) -
+                    (tiles.cuts.begin() + 1));
+    const int32_t e = rc.band_jend.empty() ? tiles.T : rc.band_jend[b];
+    jf[b] = f;
+    je[b] = e;
+    partner_tiles += std::max(0, e - f);
+  }
+  auto cut_items = [&](int32_t chunk, int32_t stride, std::vector<int4>& v) {
+    int64_t tp = 0;
+    for (int32_t b = 0; b < bands.T; b += stride)
+      for (int32_t j = jf[b]; j < je[b]; j += chunk) {
+        v.push_back(make_int4(b, j, std::min(je[b], j + chunk), 0));
+        tp += std::min(je[b], j + chunk) - j;
+      }
+    return tp;
+  };
+  const int64_t want_items = 8 * (int64_t)n_cu;
+  const int32_t chunk = (int32_t)std::max<int64_t>(
+      16, std::min<int64_t>(4096, (partner_tiles + want_items - 1) / want_items));
+  std::vector<int4> hitems;
+  cut_items(chunk, 1, hitems);
+  const int32_t n_items = (int32_t)hitems.size();
+  int4* items = ar.alloc<int4>(std::max(n_items, 1));
+  if (n_items > 0)
+    APSS_HIP(hipMemcpyAsync(items, hitems.data(), sizeof(int4) * n_items, hipMemcpyHostToDevice, st));
+  // persistent grid: one 1024-thread workgroup per CU pulls work items from a
+  // queue; each owns a cursor-state region (records of one band) and a
+  // cursor-product region large enough for every product one tile pair can
+  // receive (records x tile rows)
+  const int grid = std::max(1, std::min(n_items, n_cu));
+  const int64_t scs_cap = std::max<int64_t>(bands.max_s, 1);
+  int4* scs = ar.alloc<int4>(scs_cap * grid);
   const int64_t cprod_cap =
       bands.nS > 0 ? std::max<int64_t>((int64_t)1 << 19, bands.max_s * TILE_ROWS) : 0;
   int2* cprod = ar.alloc<int2>(cprod_cap * grid);
+
+  ScanArgs a{};
+  a.I = make_view(I, bands);
+  a.J = make_view(J, tiles);
+  a.S.ptr = tr.sptr;
+  a.S.ent = tr.sent;
+  a.S.pos = tr.spos;
+  a.KD = tr.KD;
+  a.KW = tr.KW;
+  a.p = tr.p;
+  a.self = rc.self;
+  a.head_mode = tr.p > 0 ? 1 : 0;
+  a.t = rc.t;
+  a.qscale = (float)qscale;
+  a.qinv = qinv;
+  a.q_lo = q_lo;
+  a.score_rel = 1e-6;
+  a.i_off = rc.i_off;
+  a.j_off = rc.j_off;
+  a.out_cnt = ctr + 0;
+  a.band_cnt = ctr + 1;
+  a.rescore_all = rc.rescore_all;
+  a.band_t = rc.band_t;
+  a.eps_max = eps_max;
+  a.stats = ctr + 2;
+  a.scs = scs;
+  a.scs_cap = scs_cap;
+  a.cprod = cprod;
+  a.cprod_cap = cprod_cap;
+  a.item_next = ctr + 9;
+  a.dsh_hi = dsh_hi;
+  a.dsh_lo = dsh_lo;
+  a.eps_dense = eps_dense;
+  a.c_bf = c_bf;
+  // BF16 head: emitted scores carry the stated BF16 bound (pair set exact)
+  if (tr.dm == DENSE_BF16) a.score_rel = 1e300;
+  auto launch = [&](int g) {
+    fill_u64(st, ctr, 16, 0);
+    if (a.n_items <= 0) return;
+    if (tr.dm == DENSE_I8)
+      hipLaunchKernelGGL(k_band_scan<DENSE_I8>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    else if (tr.dm == DENSE_BF16)
+      hipLaunchKernelGGL(k_band_scan<DENSE_BF16>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_band_scan<DENSE_OFF>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    APSS_HIP(hipGetLastError());
+  };
+  PhaseTrace ph;
+
+  // Output capacity.  A small pair space gets room for every cell.  Otherwise
+  // a count-only launch (no output room, so the kernel only counts) over every
+  // stride-th band against all its partner tiles, cut in short chunks so the
+  // sample fills the chip, estimates the result and band-list sizes, scaled
+  // by tile pairs; a launch that still overflows is rerun at the exact size.
+  size_t free_b = 0, total_b = 0;
+  APSS_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double room = (double)free_b * 0.4 / sizeof(apss_pair);
+  int64_t cap, bcap;
+  if (rc.out_cap_hint > 0) {
+    cap = (int64_t)std::min<double>(cells, (double)rc.out_cap_hint);
+    bcap = std::max<int64_t>(1 << 20, cap / 64);
+  } else if (cells <= (double)(1 << 24) || bands.T < 64) {
+    cap = (int64_t)std::min(cells, room);
+    bcap = std::max<int64_t>(1 << 20, std::min<int64_t>(cap, 1 << 24));
+  } else {
+    const int32_t stride = std::max<int32_t>(1, bands.T / 128);
+    std::vector<int4> sitems;
+    const int64_t stp = cut_items(64, stride, sitems);
+    int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
+    APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
+    a.items = sdev;
+    a.n_items = (int32_t)sitems.size();
+    a.out = nullptr;
+    a.out_cap = 0;
+    a.band = nullptr;
+    a.band_cap = 0;
+    launch(std::max(1, std::min(a.n_items, n_cu)));
+    unsigned long long h2[2];
+    APSS_HIP(hipMemcpyAsync(h2, ctr, sizeof(h2), hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    ar.release(sdev);
+    const double scale = stp > 0 ? (double)partner_tiles / (double)stp : 1.0;
+    cap = (int64_t)std::min(cells, std::min(room, (double)h2[0] * scale * 1.25 + (double)(1 << 22)));
+    bcap = (int64_t)((double)h2[1] * scale * 1.25) + (1 << 20);
+    ph.mark("output estimate");
+  }
+  cap = std::max<int64_t>(cap, 1 << 16);
+  a.items = items;
+  a.n_items = n_items;
+
   Timer tm(st);
+  ph.mark("scan set-up");
   for (int attempt = 0; attempt < 3; ++attempt) {
     apss_pair* out = ar.alloc<apss_pair>(cap);
     unsigned long long* band = ar.alloc<unsigned long long>(bcap);
-    fill_u64(st, ctr, 8, 0);
-    ScanArgs a{};
-    a.I = make_view(I, bands);
-    a.J = make_view(J, tiles);
-    a.S.ptr = tr.sptr;
-    a.S.ent = tr.sent;
-    a.S.pos = tr.spos;
-    a.KD = tr.KD;
-    a.KW = tr.KW;
-    a.p = tr.p;
-    a.self = rc.self;
-    a.head_mode = tr.p > 0 ? 1 : 0;
-    a.t = rc.t;
-    a.qscale = (float)qscale;
-    a.qinv = qinv;
-    a.q_lo = q_lo;
-    a.score_rel = 1e-6;
-    a.i_off = rc.i_off;
-    a.j_off = rc.j_off;
     a.out = out;
-    a.out_cnt = ctr + 0;
     a.out_cap = cap;
     a.band = band;
-    a.band_cnt = ctr + 1;
     a.band_cap = bcap;
-    a.rescore_all = rc.rescore_all;
-    a.band_jend = rc.band_jend;
-    a.band_t = rc.band_t;
-    a.eps_max = eps_max;
-    a.band_first = 0;
-    a.band_count = bands.T;
-    a.stats = ctr + 2;
-    a.scs = scs;
-    a.cprod = cprod;
-    a.cprod_cap = cprod_cap;
-    a.band_next = ctr + 7;
-    a.dsh_hi = dsh_hi;
-    a.dsh_lo = dsh_lo;
-    a.eps_dense = eps_dense;
-    a.c_bf = c_bf;
-    // BF16 head: emitted scores carry the stated BF16 bound (pair set exact)
-    if (tr.dm == DENSE_BF16) a.score_rel = 1e300;
     tm.start();
-    if (bands.T > 0 && tiles.T > 0) {
-      if (tr.dm == DENSE_I8)
-        hipLaunchKernelGGL(k_band_scan<DENSE_I8>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
-      else if (tr.dm == DENSE_BF16)
-        hipLaunchKernelGGL(k_band_scan<DENSE_BF16>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
-      else
-        hipLaunchKernelGGL(k_band_scan<DENSE_OFF>, dim3(grid), dim3(SCAN_THREADS), 0, st, a);
-      APSS_HIP(hipGetLastError());
-    }
+    launch(grid);
     so.ms_scan = tm.stop_ms();
-    unsigned long long h[8];
+    ph.mark("band scan");
+    unsigned long long h[16];
     APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     const int64_t nout = (int64_t)h[0], nband = (int64_t)h[1];
@@ -614,6 +784,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.cells = (int64_t)h[4];
     so.visits = (int64_t)h[5];
     so.cursor = (int64_t)h[6];
+    so.dense_rows = (int64_t)h[7];
+    so.tile_pairs = (int64_t)h[8];
     so.band = nband;
     so.n_dense = tr.n_dense;
     if (nout + nband > cap) {  // room for every band pair before the rescore
@@ -631,6 +803,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       APSS_HIP(hipGetLastError());
     }
     so.ms_rescore = tm.stop_ms();
+    ph.mark("rescore");
     APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     so.count = (int64_t)h[0];
@@ -678,9 +851,11 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   Arena ar(o->device, o->mem_budget);
   if (p < 0) p = 0;  // autotune picks p = 0 for the FP32 path (all-tail scan)
   Timer tm(st);
+  PhaseTrace ph;
   tm.start();
   Side s = prepare_side(ar, st, *X, p);
   out->ms_upload = tm.stop_ms();
+  ph.mark("upload + validate");
   tm.start();
   Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features, dense_mode(*o));
   TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, cap_i_for(tr.dm), CAP_S, true, {});
@@ -688,6 +863,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   set_cursor_tiles(ar, st, tr, tiles);
   build_dense(ar, st, s, tr);
   out->ms_index = tm.stop_ms();
+  ph.mark("index");
   RunConfig rc;
   rc.t = t;
   rc.p = p;
@@ -705,6 +881,8 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->list_visits = so.visits;
   out->cursor_products = so.cursor;
   out->mfma_features = so.n_dense;
+  out->dense_rows = so.dense_rows;
+  out->tile_pairs = so.tile_pairs;
   tm.start();
   if (o->output_on_device && !o->sort_output) {
     out->pairs = (apss_pair*)ar.detach(so.pairs);
@@ -729,6 +907,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
     }
   }
   out->ms_download = tm.stop_ms();
+  ph.mark("result");
   APSS_HIP(hipStreamSynchronize(st));
   out->ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -827,7 +1006,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   set_cursor_tiles(ar, st, tr, tiles);
   build_dense(ar, st, s, tr);
   // per band: instance, partner range end, threshold
-  std::vector<int32_t> hjend((size_t)std::max(bands.T, 1));
+  std::vector<int32_t> hjend((size_t)std::max(bands.T, 0));
   std::vector<double> ht((size_t)std::max(bands.T, 1));
   for (int32_t b = 0; b < bands.T; ++b) {
     const int64_t r = bands.cuts[b];
@@ -836,19 +1015,16 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     hjend[b] = (int32_t)(std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) - tiles.cuts.begin());
     ht[b] = t[k];
   }
-  int32_t* djend = ar.alloc<int32_t>(bands.T);
   double* dt = ar.alloc<double>(bands.T);
-  if (bands.T > 0) {
-    APSS_HIP(hipMemcpyAsync(djend, hjend.data(), sizeof(int32_t) * bands.T, hipMemcpyHostToDevice, st));
+  if (bands.T > 0)
     APSS_HIP(hipMemcpyAsync(dt, ht.data(), sizeof(double) * bands.T, hipMemcpyHostToDevice, st));
-  }
   const double ms_index = tm.stop_ms();
   RunConfig rc;
   rc.t = t_min;
   rc.p = p;
   rc.self = 1;
   rc.rescore_all = o->rescore_all;
-  rc.band_jend = djend;
+  rc.band_jend = std::move(hjend);
   rc.band_t = dt;
   rc.t_min = t_min;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
@@ -886,6 +1062,8 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     out[k].products = so.products;
     out[k].list_visits = so.visits;
     out[k].cursor_products = so.cursor;
+    out[k].dense_rows = so.dense_rows;
+    out[k].tile_pairs = so.tile_pairs;
     out[k].ms_upload = ms_upload;
     out[k].ms_index = ms_index;
     out[k].ms_scan = so.ms_scan;

@@ -164,14 +164,14 @@ __global__ __launch_bounds__(256) void k_sort_small(const int32_t* __restrict__ 
   if (lane < len) sent[a + lane] = e;
 }
 
-// One workgroup sorts one list of length <= S_LIST_MAX in LDS.
-__global__ __launch_bounds__(256) void k_sort_large(const int32_t* __restrict__ lists,
-                                                    const int64_t* __restrict__ sptr,
-                                                    int2* __restrict__ sent) {
+// One workgroup sorts one piece (<= S_LIST_MAX entries) of a cursor list by
+// row in LDS (bitonic network over the piece padded to a power of two).
+__global__ __launch_bounds__(256) void k_sort_chunks(const SortChunk* __restrict__ chunks,
+                                                     int2* __restrict__ sent) {
   __shared__ int2 buf[S_LIST_MAX];
-  int32_t f = lists[blockIdx.x];
-  int64_t a = sptr[f];
-  int len = (int)(sptr[f + 1] - a);
+  const SortChunk c = chunks[blockIdx.x];
+  const int64_t a = c.start;
+  const int len = c.len;
   int np = 64;
   while (np < len) np <<= 1;
   for (int i = threadIdx.x; i < np; i += blockDim.x)
@@ -191,6 +191,33 @@ __global__ __launch_bounds__(256) void k_sort_large(const int32_t* __restrict__ 
     }
   }
   for (int i = threadIdx.x; i < len; i += blockDim.x) sent[a + i] = buf[i];
+}
+
+// One merge-pass unit: 256 threads x 8 consecutive outputs of the merge of
+// two row-sorted runs.  Each thread finds where its first output splits the
+// runs (merge path: the number i of A entries among the first d outputs is
+// the first i with A[i] > B[d-i-1]), then merges sequentially.  A run pair
+// with lb = 0 is a plain copy.
+__global__ __launch_bounds__(256) void k_merge_runs(const MergeTask* __restrict__ tasks,
+                                                    const int2* __restrict__ src,
+                                                    int2* __restrict__ dst) {
+  const MergeTask t = tasks[blockIdx.x];
+  const int2* A = src + t.a;
+  const int2* B = A + t.la;
+  int2* O = dst + t.a;
+  const int total = t.la + t.lb;
+  const int d = t.k * MERGE_TILE + (int)threadIdx.x * 8;
+  if (d >= total) return;
+  int lo = d - t.lb > 0 ? d - t.lb : 0, hi = d < t.la ? d : t.la;
+  while (lo < hi) {
+    const int i = (lo + hi) >> 1;
+    if (A[i].x <= B[d - i - 1].x) lo = i + 1; else hi = i;
+  }
+  int i = lo, j = d - lo;
+  for (int q = 0; q < 8 && d + q < total; ++q) {
+    const bool take_a = j >= t.lb || (i < t.la && A[i].x <= B[j].x);
+    O[d + q] = take_a ? A[i++] : B[j++];
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -9,8 +9,12 @@
 // every corpus row).
 //
 // Inserts append a delta segment (its own tiers and tiles, row ids continuing
-// the corpus); when more than MAX_SEGMENTS segments exist they are merged into
-// one.  Queries hold a shared lock, inserts an exclusive one.
+// the corpus), built outside the query lock; when more than MAX_SEGMENTS
+// segments exist the delta segments are merged into one (and into the base
+// segment once they hold a quarter of its rows).  Queries hold a shared lock;
+// an insert takes the exclusive lock only to swap segments in.  Each serving
+// slot keeps its device blocks in a pool, so steady-state requests allocate
+// nothing with hipMalloc.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -80,9 +84,28 @@ struct Index {
   int64_t d = 0;
   apss_opts opts{};
   int64_t n_rows = 0;
+  int64_t scratch = 0;  // device bytes of one query launch's scan scratch
   std::vector<std::unique_ptr<Segment>> segs;
   mutable std::shared_mutex mu;
+  std::mutex insert_mu;  // serialises inserts (segment builds run outside mu)
 };
+
+// Upper bound of the scan scratch one query launch allocates: per workgroup
+// of the persistent grid, cursor state for a band's records and the cursor
+// product region (both sized by the band cut CAP_S).
+static int64_t scan_scratch_bytes(int device) {
+  int n_cu = 256;
+  (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  const int64_t per_wg = (int64_t)CAP_S * (int64_t)sizeof(int4) +
+                         std::max<int64_t>((int64_t)1 << 19, (int64_t)CAP_S * TILE_ROWS) * (int64_t)sizeof(int2);
+  return (int64_t)std::max(n_cu, 1) * per_wg;
+}
+
+// Output room a query batch starts with (grown and rerun if exceeded).
+static int64_t query_out_hint(int64_t q_rows, int64_t corpus_rows) {
+  return std::min<int64_t>(q_rows * std::max<int64_t>(corpus_rows, 1),
+                           std::max<int64_t>((int64_t)1 << 16, q_rows * 16384));
+}
 
 static std::unique_ptr<Segment> build_segment(HostCSR&& h, int32_t p, const apss_opts& o,
                                               int64_t row_off) {
@@ -107,10 +130,11 @@ static std::unique_ptr<Segment> build_segment(HostCSR&& h, int32_t p, const apss
 // thresholds (one per request).  Pairs (query row, corpus row) go to `pairs`.
 static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int64_t>& bounds,
                         const std::vector<double>& t_req, hipStream_t st, int64_t budget,
-                        int rescore_all, std::vector<apss_pair>& pairs, apss_result* stats) {
+                        int rescore_all, std::vector<apss_pair>& pairs, apss_result* stats,
+                        DevicePool* pool) {
   std::shared_lock<std::shared_mutex> lk(ix.mu);
   if (Q.d > ix.d) throw Error{APSS_E_INVALID, "query has more columns than the corpus"};
-  Arena ar(ix.device, budget);
+  Arena ar(ix.device, budget, pool);
   apss_csr qv = Q.view();
   qv.n_cols = ix.d;
   Timer tm(st);
@@ -145,6 +169,7 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     rc.j_off = sg->row_off;
     rc.band_t = dt;
     rc.t_min = t_min;
+    rc.out_cap_hint = query_out_hint(Q.n(), sg->side.n);
     ScanOutput so = scan_and_rescore(ar, st, qd, bands, sg->side, sg->tiles, sg->tr, rc);
     const size_t at = pairs.size();
     pairs.resize(at + (size_t)so.count);
@@ -158,6 +183,8 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
       stats->products += so.products;
       stats->list_visits += so.visits;
       stats->cursor_products += so.cursor;
+      stats->dense_rows += so.dense_rows;
+      stats->tile_pairs += so.tile_pairs;
       stats->ms_index += ms_idx;
       stats->ms_scan += so.ms_scan;
       stats->ms_rescore += so.ms_rescore;
@@ -204,10 +231,11 @@ struct Server {
 // Rows packed into one launch when several small requests wait.
 constexpr int64_t PACK_ROWS = 4096;
 
-static int64_t request_bytes(const HostCSR& q, int64_t corpus_rows) {
-  // query-side structures + output room (a generous share of the pair space)
-  const int64_t out = std::min<int64_t>(q.n() * corpus_rows, (int64_t)1 << 27) / 16 * 12;
-  return ((int64_t)64 << 20) + (int64_t)q.col.size() * 96 + q.n() * 512 + out;
+static int64_t request_bytes(const HostCSR& q, int64_t corpus_rows, int64_t scratch) {
+  // query-side structures, the output room the launch starts with (+ the band
+  // list and a regrow margin) and the scan scratch of one launch
+  const int64_t out = query_out_hint(q.n(), corpus_rows) * (int64_t)sizeof(apss_pair) * 5 / 4;
+  return ((int64_t)64 << 20) + (int64_t)q.col.size() * 96 + q.n() * 512 + out + scratch;
 }
 
 static void worker_loop(Server* s, int slot) {
@@ -216,6 +244,7 @@ static void worker_loop(Server* s, int slot) {
   if (hipSetDevice(s->idx->device) != hipSuccess ||
       hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
     st = nullptr;
+  DevicePool pool;  // this slot's reusable device blocks
   for (;;) {
     std::vector<Request*> batch;
     int64_t est = 0;
@@ -260,7 +289,7 @@ static void worker_loop(Server* s, int slot) {
     std::string err;
     try {
       if (!st) throw Error{APSS_E_DEVICE, "stream creation failed"};
-      query_index(*s->idx, Q, bounds, ts, st, est, 0, pairs, &stats);
+      query_index(*s->idx, Q, bounds, ts, st, est, 0, pairs, &stats, &pool);
     } catch (const Error& e) {
       rc = e.code;
       err = e.what;
@@ -349,6 +378,7 @@ int apss_index_create(const apss_csr* corpus, int32_t p, const apss_opts* o, aps
     ix->impl.d = corpus->n_cols;
     ix->impl.opts = *o;
     ix->impl.n_rows = corpus->n_rows;
+    ix->impl.scratch = scan_scratch_bytes(o->device);
     ix->impl.segs.push_back(build_segment(to_host(*corpus, o->device), p, *o, 0));
     *out = ix.release();
     return APSS_OK;
@@ -363,16 +393,33 @@ int apss_index_insert(apss_index* ix, const apss_csr* rows) {
     check_csr_host(*rows, std::min<int64_t>(I.p, rows->n_cols));
     HostCSR h = to_host(*rows, I.device);
     h.d = I.d;
-    std::unique_lock<std::shared_mutex> lk(I.mu);
-    I.segs.push_back(build_segment(std::move(h), I.p, I.opts, I.n_rows));
-    I.n_rows += rows->n_rows;
-    if (I.segs.size() > MAX_SEGMENTS) {  // merge every segment into one
+    // only inserts change segs and n_rows, and they hold insert_mu: reading
+    // them here needs no query lock; queries keep running while segments build
+    std::lock_guard<std::mutex> ins(I.insert_mu);
+    auto sg = build_segment(std::move(h), I.p, I.opts, I.n_rows);
+    {
+      std::unique_lock<std::shared_mutex> lk(I.mu);
+      I.segs.push_back(std::move(sg));
+      I.n_rows += rows->n_rows;
+    }
+    if (I.segs.size() > MAX_SEGMENTS) {
+      // merge the delta segments into one; fold them into the base segment
+      // once they hold a quarter of its rows
+      int64_t delta_rows = 0;
+      for (size_t k = 1; k < I.segs.size(); ++k) delta_rows += I.segs[k]->host.n();
+      const size_t first = 4 * delta_rows >= I.segs[0]->host.n() ? 0 : 1;
       HostCSR all;
       all.d = I.d;
-      for (auto& sg : I.segs) all.append(sg->host.view());
-      I.segs.clear();
-      I.segs.push_back(build_segment(std::move(all), I.p, I.opts, 0));
-    }
+      for (size_t k = first; k < I.segs.size(); ++k) all.append(I.segs[k]->host.view());
+      auto merged = build_segment(std::move(all), I.p, I.opts, I.segs[first]->row_off);
+      std::vector<std::unique_ptr<Segment>> old;
+      {
+        std::unique_lock<std::shared_mutex> lk(I.mu);
+        for (size_t k = first; k < I.segs.size(); ++k) old.push_back(std::move(I.segs[k]));
+        I.segs.resize(first);
+        I.segs.push_back(std::move(merged));
+      }
+    }  // the replaced segments are freed here, outside the lock
     return APSS_OK;
   });
 }
@@ -388,7 +435,7 @@ int apss_index_query(apss_index* ix, const apss_csr* queries, double t, apss_res
     HostCSR q = to_host(*queries, I.device);
     StreamScope ss(I.opts.stream, I.device);
     std::vector<apss_pair> pairs;
-    query_index(I, q, {}, {t}, ss.s, I.opts.mem_budget, I.opts.rescore_all, pairs, out);
+    query_index(I, q, {}, {t}, ss.s, I.opts.mem_budget, I.opts.rescore_all, pairs, out, nullptr);
     if (I.opts.sort_output) sort_pairs(pairs.data(), (int64_t)pairs.size());
     out->count = (int64_t)pairs.size();
     out->pairs = (apss_pair*)std::malloc(sizeof(apss_pair) * std::max<size_t>(pairs.size(), 1));
@@ -445,7 +492,7 @@ int64_t apss_server_submit(apss_server* s, const apss_csr* queries, double t) {
     r->q = to_host(*queries, S.idx->device);
     r->q.d = S.idx->d;
     r->t = t;
-    r->est = request_bytes(r->q, S.idx->n_rows);
+    r->est = request_bytes(r->q, S.idx->n_rows, S.idx->scratch);
     r->t_submit = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> lk(S.mu);
     if (S.stop) return APSS_E_CANCELLED;

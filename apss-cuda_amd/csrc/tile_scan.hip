@@ -238,35 +238,44 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ uint4 dA[DM ? BAND_ROWS * 16 : 1];      // head records of the band
   __shared__ uint4 dB[DM ? TILE_ROWS * 16 : 1];      // head records of tile J
   __shared__ float dnI[DM == DENSE_BF16 ? BAND_ROWS : 1], dnJ[DM == DENSE_BF16 ? TILE_ROWS : 1];
-  __shared__ int32_t band_s, J_first_s, ch_end_s;
+  __shared__ int4 item_s;
+  __shared__ int32_t ch_end_s;
   __shared__ unsigned int cand_n;                    // candidate cells of the tile pair
   __shared__ unsigned int rec_next;                  // cursor records handed out so far
   __shared__ unsigned long long acc_products, acc_head, acc_cells, acc_visits, acc_cursor;
+  __shared__ unsigned long long acc_dense, acc_tp;
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
   static_assert(BAND_ROWS == 128 && TILE_ROWS == 64 && SCAN_THREADS == 1024,
                 "MFMA layout: 16 waves x two 16x16 blocks cover the 128 x 64 tile");
 
   const int tid0 = threadIdx.x;
   int2* const cprod = a.cprod + (int64_t)blockIdx.x * a.cprod_cap;
+  int4* const scs = a.scs + (int64_t)blockIdx.x * a.scs_cap;
 
   // the tile is cleared cell by cell by the threshold pass, so once is enough
   for (int c = tid0; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
     acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
+    acc_dense = 0; acc_tp = 0;
   }
   unsigned int my_products = 0, my_visits = 0;
 
   for (;;) {
   int tid = tid0;
   asm volatile("" : "+v"(tid));
-  if (tid == 0) band_s = a.band_first + (int32_t)atomicAdd(a.band_next, 1ull);
+  // work item: band I against the partner tiles [J_first, J_end)
+  if (tid == 0) {
+    const unsigned long long w = atomicAdd(a.item_next, 1ull);
+    item_s = w < (unsigned long long)a.n_items ? a.items[w] : make_int4(-1, 0, 0, 0);
+  }
   __syncthreads();
-  const int32_t I = band_s;
-  if (I >= a.band_first + a.band_count || I >= a.I.T) break;
+  const int4 item = item_s;
+  if (item.x < 0) break;
+  const int32_t I = item.x;
   const int64_t i0 = a.I.tstart[I];
   const int nI = (int)(a.I.tstart[I + 1] - i0);
-  // batched runs give each band its own partner range and threshold
-  const int32_t J_end = a.band_jend ? a.band_jend[I] : a.J.T;
+  const int32_t J_first = item.y, J_end = item.z;
+  // batched runs and packed queries give each band its own threshold
   const double t_band = a.band_t ? a.band_t[I] : a.t;
   int32_t q_lo = a.q_lo;
   if (a.band_t) {
@@ -308,27 +317,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       dA[dslot(row, c & 15)] = row < nI ? a.I.dense[(i0 + row) * 16 + (c & 15)] : make_uint4(0, 0, 0, 0);
     }
   }
-  if (tid == 0) {
-    int32_t jf = 0;
-    if (a.self) {  // first tile whose rows end after the band's first row
-      int32_t lo = 0, hi = a.J.T;
-      while (lo < hi) {
-        int32_t m = (lo + hi) >> 1;
-        if (a.J.tstart[m + 1] <= i0) lo = m + 1; else hi = m;
-      }
-      jf = lo;
-    }
-    J_first_s = jf;
-  }
-  // cursor records: position and the J tile it points into
+  // cursor records: the first position whose J tile is >= J_first (lists are
+  // row-sorted, so positions are ascending in tile << 8 | row) and its tile
   const int64_t sb = a.I.sbase[I];
   const int nS = (int)(a.I.sbase[I + 1] - sb);
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     const int4 r = a.I.srec[sb + k];
-    a.scs[sb + k] = make_int4(r.z, r.w, r.z < r.w ? (int32_t)(a.S.pos[r.z] >> 8) : INT32_MAX, 0);
+    int lo = r.z, hi = r.w;
+    const uint32_t key = (uint32_t)J_first << 8;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (a.S.pos[m] < key) lo = m + 1; else hi = m;
+    }
+    scs[k] = make_int4(lo, r.w, lo < r.w ? (int32_t)(a.S.pos[lo] >> 8) : INT32_MAX, 0);
   }
   __syncthreads();
-  const int32_t J_first = J_first_s;
 
   Prefetch<DM> pf;
   if (J_first < J_end) prefetch_tile<DM>(a.J, J_first, tid, pf);
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (;;) {
         const int k = (int)atomicAdd(&rec_next, 1u);
         if (k >= nS) break;
-        const int4 st = a.scs[sb + k];
+        const int4 st = scs[k];
         if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt);
       }
       __syncthreads();
@@ -373,14 +376,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (;;) {
         const int k = (int)atomicAdd(&rec_next, 1u);
         if (k >= nS) break;
-        const int4 st = a.scs[sb + k];
+        const int4 st = scs[k];
         if (st.z >= ce) continue;
         const int4 r = a.I.srec[sb + k];
         int32_t tn;
         unsigned int np = 0;
         const int cur = walk_scatter(a.S.pos, a.S.ent, st.x, st.y, ch_lo, ce, r.x * TSTRIDE,
                                      __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn);
-        a.scs[sb + k] = make_int4(cur, st.y, tn, 0);
+        scs[k] = make_int4(cur, st.y, tn, 0);
         if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
       }
       __syncthreads();
@@ -590,6 +593,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         c = G(dj + nJ) - G(dj);
       }
       acc_cells += (unsigned long long)c;
+      acc_tp += 1;
+      // head records: the tile's rows per tile pair, the band's once per item
+      if (DM != DENSE_OFF) acc_dense += (unsigned long long)(nJ + (J == J_first ? nI : 0));
     }
     // Cells past the band or tile height receive no products and a zero head,
     // so with q_lo > 0 only the self-join diagonal needs an explicit validity
@@ -742,6 +748,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     atomicAdd(&a.stats[2], acc_cells);
     atomicAdd(&a.stats[3], acc_visits);
     atomicAdd(&a.stats[4], acc_cursor);
+    atomicAdd(&a.stats[5], acc_dense);
+    atomicAdd(&a.stats[6], acc_tp);
   }
 }
 

@@ -1,30 +1,47 @@
 """APSS benchmark on one MI355X.
 
-Workload: the large synthetic configuration -- n = 1,000,000 rows,
+Headline workload: the large synthetic configuration -- n = 1,000,000 rows,
 d = 1,000,000 features, ~1e8 nonzeros (power-law row lengths, mean 100, in
 [5, 5000]), Zipf(1.05) feature ids, lognormal(0,1) values, L2-normalised --
-self-joined at t = 0.5 with split p = 0 (every feature in the inverted-list
-tiers, the fastest FP32 form).
+self-joined at t = 0.5 with split p = 0 (every feature on the inverted-list /
+dense-head path), FP32 mode (pair set identical to the FP64 result, scores
+within 1e-6 relative).
 
 A step is one full APSS job on the device-resident CSR: validation + norms,
-feature histogram and tier selection, tiled inverted index, the band-scan
-kernel over all tile pairs, the FP64 rescore of the error band; the pair list
-stays in device memory.  value = n(n-1)/2 / wall (pair-space throughput, the
-same for every t); result pairs/s and a t = 0.9 step are reported beside it.
+feature histogram and tier selection, tiled inverted index and dense head
+records, the band-scan kernel over all tile pairs, the FP64 rescore of the
+error band; the pair list stays in device memory.  value = n(n-1)/2 / wall
+(pair-space throughput, comparable across t); result pairs/s beside it.
 
-Extras on the JSON line:
-  roofline      band-scan kernel: algorithmic bytes (list-entry visits of both
-                tiles for every common feature, 8 B each; cursor-tier products,
-                16 B each; candidate records leaving the tile, 12 B each) over
-                its hipEvent-timed duration, against 8 TB/s;
-  cpu_baseline  the reference algorithm restated for the CPU (oracle/), timed
-                on the host cores on a stratified sample of query rows, each
-                joined with every later row (pair-space throughput).
+Legs printed on the same JSON line (each can be switched off):
+  t0.9      one step of the large workload at t = 0.9;
+  variants  the large workload at t = 0.5 without the dense MFMA head (the
+            plain sparse path) and with the BF16 head (mixed precision);
+  mixed     the mixed-precision configuration (n = d = 5e5, mean 80 nnz/row,
+            Zipf(1.2), uniform values, t = 0.5): plain path, exact int8 head,
+            BF16 head -- the pair sets must be identical;
+  batched   256 independent instances (n log-uniform in [500, 5e4], d = 1e5)
+            in one band-scan launch, against the same instances run one by one;
+  serving   2e6-row resident corpus (d = 1e6, mean 60 nnz/row, Zipf(1.1),
+            lognormal), query batches of 1..4096 rows (log-uniform), t drawn
+            from {0.6, 0.8}, Poisson arrivals at fractions of the measured
+            capacity, 1% of the submitted rows inserted into the index while
+            queries run; p50 / p99 request latency (submit -> result);
+  roofline  band-scan kernel: algorithmic bytes (list-entry visits of both
+            tiles for every common feature, 8 B each; cursor-tier products,
+            16 B each; dense head records fetched, 256 B each; pairs leaving
+            the tile, 12 B each) over its hipEvent-timed duration on the
+            stream it runs on, against 8 TB/s;
+  cpu       the reference algorithm restated for the CPU (oracle/), timed on
+            the host cores on a stratified sample of query rows, each joined
+            with every later row (pair-space throughput).
 """
 import argparse
 import json
+import math
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -35,6 +52,7 @@ import numpy as np  # noqa: E402
 import apss  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+ALL_LEGS = ("t09", "variants", "mixed", "batched", "serving", "cpu")
 
 
 def log(msg):
@@ -51,6 +69,12 @@ def traffic_from_profiles():
         return None
 
 
+def alg_bytes(info):
+    """Algorithmic bytes of one band-scan launch from its counters."""
+    return (8.0 * info["list_visits"] + 16.0 * info["cursor_products"] + 256.0 * info["dense_rows"]
+            + 12.0 * (info["count"] + info["band"]))
+
+
 def cpu_baseline(rp, col, val, d, t, budget_s):
     """Reference hybrid (oracle/, FP32, p = 4096 as in the reference) on a
     stratified row sample; returns the pair-space throughput dict."""
@@ -59,11 +83,12 @@ def cpu_baseline(rp, col, val, d, t, budget_s):
 
     n = len(rp) - 1
     threads = oracle.num_threads()
-    # calibrate on a small sample, then size the sample to the time budget
-    probe = np.linspace(0, n - 2, num=max(threads, 8), dtype=np.int64)
+    # calibrate on a small sample that keeps every thread busy (the restatement
+    # hands out query rows in blocks of 16), then size the sample to the budget
+    probe = np.linspace(0, n - 2, num=min(n - 1, 32 * threads), dtype=np.int64)
     _, st = oracle.hybrid_fp32(rp, col, val, d, t, 4096, probe)
-    per_row = max(st["search_seconds"], 1e-6) * threads / len(probe)
-    m = int(min(n - 1, max(threads, budget_s * threads / per_row)))
+    wall_per_row = max(st["search_seconds"], 1e-6) / len(probe)
+    m = int(min(n - 1, max(16 * threads, budget_s / wall_per_row)))
     rows = np.linspace(0, n - 2, num=m, dtype=np.int64)
     _, st = oracle.hybrid_fp32(rp, col, val, d, t, 4096, rows)
     pairs = float(np.sum((n - 1 - rows).astype(np.float64)))
@@ -77,6 +102,178 @@ def cpu_baseline(rp, col, val, d, t, budget_s):
     }
 
 
+def info_of(r):
+    return {f: getattr(r, f) for f, _ in apss.Result._fields_ if f not in ("pairs", "internal_")}
+
+
+def run_device(Xc, t, p, stream, **opts):
+    r = apss.run_csr(Xc, t, p, output_on_device=1, stream=stream, **opts)
+    info = info_of(r)
+    apss.lib().apss_result_free(apss.ctypes.byref(r))
+    return info
+
+
+def leg_mixed(hip, stream):
+    """Mixed-precision configuration: plain sparse path vs exact int8 head vs BF16 head."""
+    t0 = time.time()
+    rp, col, val, d = apss.workload("mixed")
+    n = len(rp) - 1
+    log(f"mixed: generated n={n} nnz={len(col)} in {time.time() - t0:.1f}s")
+    X = apss.DeviceCSR(hip, rp, col, val, d)
+    Xc = X.csr()
+    out = {"workload": f"mixed: n={n} d={d} nnz={len(col)} Zipf(1.2) uniform, t=0.5, p=0"}
+    forms = (("plain", dict(mfma_head=-1)), ("int8_head", dict(mfma_head=0)),
+             ("bf16_head", dict(mfma_head=0, precision=apss.PREC_BF16_HEAD)))
+    counts = {}
+    for name, o in forms:
+        run_device(Xc, 0.5, 0, stream, **o)  # warm-up (first-call costs)
+        info = run_device(Xc, 0.5, 0, stream, **o)
+        counts[name] = info["count"]
+        out[name] = {"wall_ms": round(info["ms_total"], 1), "scan_ms": round(info["ms_scan"], 1),
+                     "pairs": info["count"], "rescored": info["band"], "mfma_features": info["mfma_features"]}
+        log(f"mixed {name}: {info['ms_total']:.1f} ms, {info['count']} pairs")
+    out["pair_sets_equal_counts"] = len(set(counts.values())) == 1
+    out["bf16_speedup_vs_plain"] = round(out["plain"]["wall_ms"] / out["bf16_head"]["wall_ms"], 3)
+    out["int8_speedup_vs_plain"] = round(out["plain"]["wall_ms"] / out["int8_head"]["wall_ms"], 3)
+    X.release()
+    return out
+
+
+def leg_batched():
+    """256 instances in one launch sequence vs the same instances one by one."""
+    t0 = time.time()
+    inst = apss.batched_instances()
+    rows = sum(len(x[0]) - 1 for x in inst)
+    nnz = sum(len(x[1]) for x in inst)
+    cells = sum((len(x[0]) - 1) * (len(x[0]) - 2) / 2.0 for x in inst)
+    log(f"batched: generated {len(inst)} instances, {rows} rows, {nnz} nnz in {time.time() - t0:.1f}s")
+    ts = [0.8] * len(inst)
+    apss.run_batched(inst[:4], ts[:4])  # warm-up
+    t1 = time.perf_counter()
+    res = apss.run_batched(inst, ts)
+    batched_s = time.perf_counter() - t1
+    pairs_b = sum(info["count"] for _, info in res)
+    t1 = time.perf_counter()
+    pairs_s = 0
+    for x in inst:
+        _, info = apss.run(*x, 0.8)
+        pairs_s += info["count"]
+    single_s = time.perf_counter() - t1
+    return {"instances": len(inst), "rows": rows, "nnz": nnz, "pair_cells": cells, "t": 0.8,
+            "batched_wall_ms": round(batched_s * 1e3, 1), "one_by_one_wall_ms": round(single_s * 1e3, 1),
+            "speedup": round(single_s / batched_s, 3), "pairs": pairs_b, "pairs_equal": pairs_b == pairs_s,
+            "pair_cells_per_s": cells / batched_s,
+            "scan_ms": round(res[0][1]["ms_scan"], 1) if res else 0.0}
+
+
+def leg_serving(args):
+    """Continuous-batching server over a resident corpus under Poisson load."""
+    t0 = time.time()
+    rp, col, val, d = apss.workload("serving", n=args.serving_rows)
+    n = len(rp) - 1
+    log(f"serving: generated corpus n={n} nnz={len(col)} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ix = apss.Index(rp, col, val, d, p=0)
+    build_s = time.time() - t0
+    log(f"serving: index built in {build_s:.1f}s")
+    del rp, col, val
+    pool_n = 1 << 16
+    qrp, qcol, qval, _ = apss.workload("serving", n=pool_n, seed_offset=1000)
+    irp, icol, ival, _ = apss.workload("serving", n=pool_n, seed_offset=2000)
+    rng = np.random.default_rng(0x5E4)
+
+    def rows_slice(arp, acol, aval, a, b):
+        return ((arp[a:b + 1] - arp[a]).astype(np.int64), acol[arp[a]:arp[b]].copy(), aval[arp[a]:arp[b]].copy())
+
+    def make_batch():
+        size = int(round(math.exp(rng.uniform(0.0, math.log(4096.0)))))
+        a = int(rng.integers(0, pool_n - size))
+        return rows_slice(qrp, qcol, qval, a, a + size), float(rng.choice([0.6, 0.8])), size
+
+    srv = apss.Server(ix, slots=args.slots, mem_budget=args.serving_budget)
+    out = {"corpus_rows": n, "features": d, "index_build_s": round(build_s, 2), "slots": args.slots,
+           "mem_budget": args.serving_budget}
+    try:
+        # capacity: closed loop, one request at a time
+        lat, rows_done, t1 = [], 0, time.perf_counter()
+        for _ in range(args.serving_probe):
+            q, t, size = make_batch()
+            tk = srv.submit(*q, t)
+            srv.wait(tk)
+            rows_done += size
+        closed_s = time.perf_counter() - t1
+        cap = args.serving_probe / closed_s
+        out["capacity_batches_per_s"] = round(cap, 2)
+        out["closed_loop_rows_per_s"] = round(rows_done / closed_s, 1)
+        log(f"serving: capacity {cap:.1f} batches/s")
+        ins_next = [0]
+        sweeps = []
+        for frac in args.serving_rates:
+            rate = frac * cap
+            nreq = int(min(args.serving_max, max(args.serving_min, rate * args.serving_seconds)))
+            batches = [make_batch() for _ in range(nreq)]
+            gaps = rng.exponential(1.0 / rate, size=nreq)
+            tickets, lats, errs = [], [], []
+            done = threading.Event()
+            cv = threading.Condition()
+
+            def waiter():
+                k = 0
+                while True:
+                    with cv:
+                        while k >= len(tickets) and not done.is_set():
+                            cv.wait(0.05)
+                        if k >= len(tickets) and done.is_set():
+                            return
+                        tk = tickets[k]
+                    k += 1
+                    try:
+                        _, ms = srv.wait(tk)
+                        lats.append(ms)
+                    except apss.ApssError as e:
+                        errs.append(e.code)
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            ins_rows, sub_rows, inserts = 0, 0, 0
+            t1 = time.perf_counter()
+            due = t1
+            for (q, t, size), g in zip(batches, gaps):
+                due += g
+                now = time.perf_counter()
+                if due > now:
+                    time.sleep(due - now)
+                tk = srv.submit(*q, t)
+                with cv:
+                    tickets.append(tk)
+                    cv.notify()
+                sub_rows += size
+                # 1% of the submitted rows are inserted into the corpus (256-row chunks)
+                while (sub_rows // 100) - ins_rows >= 256:
+                    a = ins_next[0] % (pool_n - 256)
+                    ix.insert(*rows_slice(irp, icol, ival, a, a + 256))
+                    ins_next[0] += 256
+                    ins_rows += 256
+                    inserts += 1
+            done.set()
+            th.join()
+            wall = time.perf_counter() - t1
+            la = np.array(lats) if lats else np.zeros(1)
+            sw = {"load": frac, "offered_batches_per_s": round(rate, 2), "requests": nreq, "completed": len(lats),
+                  "errors": len(errs), "p50_ms": round(float(np.percentile(la, 50)), 2),
+                  "p99_ms": round(float(np.percentile(la, 99)), 2), "mean_ms": round(float(la.mean()), 2),
+                  "throughput_batches_per_s": round(len(lats) / wall, 2),
+                  "query_rows_per_s": round(sub_rows / wall, 1), "inserted_rows": ins_rows, "inserts": inserts}
+            sweeps.append(sw)
+            log(f"serving load {frac}: p50 {sw['p50_ms']} ms p99 {sw['p99_ms']} ms, {sw['throughput_batches_per_s']} batches/s")
+        out["sweeps"] = sweeps
+        out["index_rows_after"] = ix.rows()
+    finally:
+        srv.close()
+        ix.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,14 +282,24 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000, help="rows (default: the full large workload)")
     ap.add_argument("--t", type=float, default=0.5)
     ap.add_argument("--p", type=int, default=0)
+    ap.add_argument("--mfma-head", type=int, default=0, help="0: dense MFMA head on, -1: off")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-t09", action="store_true")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: " + ",".join(ALL_LEGS))
+    ap.add_argument("--only-main", action="store_true", help="skip every leg beside the headline steps")
+    ap.add_argument("--slots", type=int, default=2)
+    ap.add_argument("--serving-rows", type=int, default=2_000_000)
+    ap.add_argument("--serving-budget", type=int, default=64 << 30)
+    ap.add_argument("--serving-probe", type=int, default=24)
+    ap.add_argument("--serving-rates", type=float, nargs="*", default=[0.5, 0.9])
+    ap.add_argument("--serving-seconds", type=float, default=8.0)
+    ap.add_argument("--serving-min", type=int, default=40)
+    ap.add_argument("--serving-max", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the tile-pair space shards across GPUs by bands; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    skip = set(ALL_LEGS) if args.only_main else {s for s in args.skip.split(",") if s}
 
     t0 = time.time()
     rp, col, val, d = apss.workload("large", n=args.n)
@@ -104,21 +311,16 @@ def main():
     stream = hip.stream()
     e0, e1 = hip.event(), hip.event()
     Xc = X.csr()
-
-    def step(t):
-        r = apss.run_csr(Xc, t, args.p, output_on_device=1, stream=stream)
-        info = {f: getattr(r, f) for f, _ in apss.Result._fields_ if f not in ("pairs", "internal_")}
-        apss.lib().apss_result_free(apss.ctypes.byref(r))
-        return info
+    opts = dict(mfma_head=args.mfma_head)
 
     for w in range(args.warmup):
-        info = step(args.t)
+        info = run_device(Xc, args.t, args.p, stream, **opts)
         log(f"warmup {w}: {info['ms_total']:.1f} ms, pairs {info['count']}")
     hip.sync()
     hip.record(e0, stream)
     infos = []
     for s in range(args.steps):
-        infos.append(step(args.t))
+        infos.append(run_device(Xc, args.t, args.p, stream, **opts))
         log(f"step {s}: total {infos[-1]['ms_total']:.1f} ms scan {infos[-1]['ms_scan']:.1f} ms")
     hip.record(e1, stream)
     hip.sync()
@@ -127,25 +329,51 @@ def main():
     cells = n * (n - 1) / 2.0
 
     # band-scan roofline: algorithmic bytes / hipEvent kernel time (same stream)
-    alg_bytes = 8.0 * last["list_visits"] + 16.0 * last["cursor_products"] + 12.0 * (last["count"] + last["band"])
+    ab = alg_bytes(last)
     scan_ms = float(np.mean([i["ms_scan"] for i in infos]))
-    achieved = alg_bytes / (scan_ms * 1e-3) / 1e9
+    achieved = ab / (scan_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                "kernel": "k_band_scan", "kernel_ms": round(scan_ms, 2), "alg_bytes_per_launch": alg_bytes}
+                "kernel": "k_band_scan", "kernel_ms": round(scan_ms, 2), "alg_bytes_per_launch": ab,
+                "terms": {"list_visits": last["list_visits"], "cursor_products": last["cursor_products"],
+                          "dense_rows": last["dense_rows"], "pairs_out": last["count"] + last["band"],
+                          "tile_pairs": last["tile_pairs"]}}
 
     extra = {}
-    if not args.no_t09:
-        info9 = step(0.9)
-        extra["t0.9"] = {"wall_ms": round(info9["ms_total"], 1), "pairs": info9["count"],
-                         "pairs_per_s": cells / (info9["ms_total"] * 1e-3)}
+    if "t09" not in skip:
+        info9 = run_device(Xc, 0.9, args.p, stream, **opts)
+        extra["t0.9"] = {"wall_ms": round(info9["ms_total"], 1), "scan_ms": round(info9["ms_scan"], 1),
+                         "pairs": info9["count"], "pairs_per_s": cells / (info9["ms_total"] * 1e-3)}
+        log(f"t=0.9: {info9['ms_total']:.1f} ms, {info9['count']} pairs")
+    if "variants" not in skip:
+        v = {}
+        for name, o in (("plain", dict(mfma_head=-1)),
+                        ("bf16_head", dict(mfma_head=0, precision=apss.PREC_BF16_HEAD))):
+            iv = run_device(Xc, args.t, args.p, stream, **o)
+            v[name] = {"wall_ms": round(iv["ms_total"], 1), "scan_ms": round(iv["ms_scan"], 1),
+                       "pairs": iv["count"], "pairs_equal": iv["count"] == last["count"],
+                       "pairs_per_s": cells / (iv["ms_total"] * 1e-3)}
+            log(f"variant {name}: {iv['ms_total']:.1f} ms, {iv['count']} pairs")
+        v["headline_speedup_vs_plain"] = round(v["plain"]["wall_ms"] / ms, 3)
+        extra["large_variants"] = v
 
     cpu = None
-    if not args.no_cpu:
+    if "cpu" not in skip:
         log("cpu baseline ...")
         cpu = cpu_baseline(rp, col, val, d, args.t, args.cpu_seconds)
-
     X.release()
+    del rp, col, val
+
+    for name, fn in (("mixed", lambda: leg_mixed(hip, stream)), ("batched", leg_batched),
+                     ("serving", lambda: leg_serving(args))):
+        if name in skip:
+            continue
+        try:
+            extra[name] = fn()
+        except Exception as e:  # noqa: BLE001 -- a failed leg is reported, the line still prints
+            log(f"{name} leg failed: {e!r}")
+            extra[name] = {"error": repr(e)}
+
     out = {
         "metric": "APSS pair-space throughput n(n-1)/2 / wall, t=0.5",
         "value": cells / (ms * 1e-3),
@@ -160,7 +388,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"large: n={n} d={d} nnz={nnz} Zipf(1.05) lognormal, t={args.t}, p={args.p}",
-                   "rows": n, "features": d, "nnz": nnz, "t": args.t, "p": args.p},
+                   "rows": n, "features": d, "nnz": nnz, "t": args.t, "p": args.p,
+                   "mfma_head": "int8-limb exact" if args.mfma_head == 0 else "off"},
         "result_pairs": last["count"],
         "result_pairs_per_s": last["count"] / (ms * 1e-3),
         "phases_ms": {k: round(last[k], 2) for k in ("ms_upload", "ms_index", "ms_scan", "ms_rescore")},

@@ -109,6 +109,8 @@ typedef struct {
   int64_t list_visits;     /* inverted-list entries visited by tile pairs (both sides) */
   int64_t cursor_products; /* products of the short-list (cursor) tier */
   int64_t mfma_features;   /* features in the dense MFMA head (0: none) */
+  int64_t dense_rows;      /* 256-byte dense head records fetched by the scan */
+  int64_t tile_pairs;      /* band x tile pairs the scan visited */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   void* internal_;

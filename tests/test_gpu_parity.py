@@ -74,6 +74,29 @@ def test_cursor_tier_forced(cap):
     run_and_check(rp, col, val, d, 0.4, 0, dense_features=cap)
 
 
+def test_long_cursor_lists_merge_sorted():
+    # three features in every one of 30000 rows: with a one-feature bitmap tier
+    # and no dense head their lists (30000 entries) go to the cursor tier, are
+    # sorted in 8192-entry pieces and merged in two passes
+    rng = np.random.default_rng(11)
+    n, d = 30000, 2000
+    rows = []
+    for _ in range(n):
+        extra = rng.choice(np.arange(3, d), size=3, replace=False)
+        cols = np.concatenate([[0, 1, 2], np.sort(extra)])
+        v = rng.uniform(0.05, 1.0, size=len(cols))
+        rows.append(dict(zip(cols.tolist(), (v / np.linalg.norm(v)).astype(np.float32).tolist())))
+    rp, col, val, d = tiny_matrix(rows, d)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.9, 0, dense_features=1, mfma_head=-1)
+    assert info["cursor_products"] > 0 and len(exp) > 0
+
+
+def test_full_small_config_long_lists():
+    # the full small configuration: the most frequent lists exceed 8192 entries
+    rp, col, val, d = apss.workload("small")
+    run_and_check(rp, col, val, d, 0.7, 0, dense_features=1, mfma_head=-1)
+
+
 def test_multi_tile_lognormal():
     rp, col, val, d = apss.workload("large", n=6000)
     got, exp, info, err = run_and_check(rp, col, val, d, 0.5, 0)

@@ -35,6 +35,17 @@ def test_batched_with_split_and_small_bitmap_tier():
         check_pairs(pairs, oracle.apss_fp64(rp, col, val, d, t))
 
 
+def test_batched_long_lists_across_instances():
+    # instances of the batched workload (scaled down): concatenated, nine of
+    # their common features form lists longer than one sort piece; a
+    # one-feature bitmap tier leaves eight of those to the cursor tier
+    inst = apss.batched_instances(count=6, n_lo=3000, n_hi=9000)
+    ts = [0.8] * len(inst)
+    res = apss.run_batched(inst, ts, p=0, dense_features=1, mfma_head=-1)
+    for (rp, col, val, d), t, (pairs, _) in zip(inst, ts, res):
+        check_pairs(pairs, oracle.apss_fp64(rp, col, val, d, t))
+
+
 @pytest.mark.parametrize("p", [0, 64])
 def test_index_query_matches_oracle(p):
     crp, ccol, cval, d = small_rows(3000, name="serving")
