@@ -286,6 +286,10 @@ class Hip:
         self._ok(self.L.hipMemcpy(c_void_p(arr.ctypes.data), c_void_p(src), ctypes.c_size_t(arr.nbytes), 2),
                  "hipMemcpy D2H")
 
+    def d2d(self, dst, src, nbytes):
+        self._ok(self.L.hipMemcpy(c_void_p(dst), c_void_p(src), ctypes.c_size_t(int(nbytes)), 3),
+                 "hipMemcpy D2D")
+
     def sync(self):
         self._ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 

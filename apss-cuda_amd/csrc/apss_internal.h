@@ -23,10 +23,13 @@ namespace apss {
 // row holds at most KD_MAX <= CAP_J bitmap-tier entries, a single row fits.
 constexpr int BAND_ROWS = 128;
 constexpr int TILE_ROWS = 64;
-// Row stride of the score tile: 68 words puts the four row groups that one
-// wave reads in the MFMA output layout (rows 4 apart, 16 consecutive columns
-// each) in disjoint bank ranges.
-constexpr int TSTRIDE = 68;
+// The score tile is stored transposed: cell (il, jl) at jl * TSTRIDE + il.
+// A lane's four cells of an MFMA output block (four consecutive band rows of
+// one tile row) are then one 16-byte word, and the 16 tile rows one wave
+// touches (stride 132 words = 4 banks apart) cover all 64 banks once; the
+// sparse products of one feature (distinct band rows, same tile row) fall in
+// distinct banks.
+constexpr int TSTRIDE = BAND_ROWS + 4;
 constexpr int SCAN_THREADS = 1024;           // band kernel workgroup (16 waves)
 constexpr int KD_MAX = 4032;                 // features in the bitmap tier (KD + 1 <= 4 x 1024)
 constexpr int KW_MAX = KD_MAX / 64;          // bitmap words per tile
@@ -165,7 +168,7 @@ struct ScanArgs {
   int64_t scs_cap;
   // cursor-tier products, bucketed by J tile for one chunk of CH_TILES tiles;
   // one region of cprod_cap records per workgroup of the persistent grid
-  int2* cprod;               // {cell = il * TSTRIDE + jl, fixed-point product}
+  int2* cprod;               // {cell = jl * TSTRIDE + il, fixed-point product}
   int64_t cprod_cap;
   unsigned long long* item_next;  // work-item queue of the persistent grid
   // dense head: int8 limb groups combine as hi << dsh_hi + (lo >> dsh_lo);

@@ -27,6 +27,7 @@ __global__ void k_scan_i64(int64_t* a, int64_t n);
 __global__ void k_tile_ptr_scan(BuildSide s);
 __global__ void k_tile_scatter(BuildSide s);
 __global__ void k_tile_compact(BuildSide s);
+__global__ void k_sort_records(const int64_t* sbase, int4* srec);
 __global__ void k_row2tile(const int64_t* tstart, int32_t* row2tile);
 __global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, const float* val,
                              int32_t p, const int32_t* fmap, int32_t dm, int32_t B, uint32_t* dense,

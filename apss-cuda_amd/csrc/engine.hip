@@ -537,6 +537,8 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   if (T > 0) {
     hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_tile_compact, dim3(T), dim3(256), 0, st, b);
+    if (records && ts.nS > 0)
+      hipLaunchKernelGGL(k_sort_records, dim3(T), dim3(256), 0, st, ts.sbase, ts.srec);
     APSS_HIP(hipGetLastError());
   }
   APSS_HIP(hipStreamSynchronize(st));

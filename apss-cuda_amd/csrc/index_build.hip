@@ -385,6 +385,42 @@ __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
   }
 }
 
+// Order the cursor records of each band by the length class of their list
+// (floor(log2(end - first)), longest first).  The band kernel hands records to
+// lanes in this order, so the 64 walks of a wave are of similar length instead
+// of every wave waiting for its longest walk.  Order inside a class is
+// arbitrary (the scatter above is too).  Bands with more records than the LDS
+// stage holds keep their order.
+__global__ __launch_bounds__(256) void k_sort_records(const int64_t* __restrict__ sbase,
+                                                      int4* __restrict__ srec) {
+  __shared__ int4 buf[CAP_S];
+  __shared__ unsigned int cnt[32], fill[32];
+  const int band = blockIdx.x;
+  const int64_t b0 = sbase[band];
+  const int nS = (int)(sbase[band + 1] - b0);
+  if (nS <= 1 || nS > CAP_S) return;
+  if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < nS; k += blockDim.x) {
+    const int4 r = srec[b0 + k];
+    buf[k] = r;
+    const int len = r.w - r.z;
+    atomicAdd(&cnt[len > 0 ? 31 - __clz(len) : 0], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // classes in descending order
+    unsigned int acc = 0;
+    for (int c = 31; c >= 0; --c) { fill[c] = acc; acc += cnt[c]; }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nS; k += blockDim.x) {
+    const int4 r = buf[k];
+    const int len = r.w - r.z;
+    const unsigned int slot = atomicAdd(&fill[len > 0 ? 31 - __clz(len) : 0], 1u);
+    srec[b0 + slot] = r;
+  }
+}
+
 // Compact the pointer row of each tile: segment starts of the present
 // features (in id order) and the per-word prefix counts of the bitmap.
 __global__ __launch_bounds__(256) void k_tile_compact(BuildSide s) {

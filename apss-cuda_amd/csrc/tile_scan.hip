@@ -65,49 +65,72 @@ struct Sizes {
   static_assert(KDM + 1 <= 4 * SCAN_THREADS, "one 4-entry chunk of segment starts per thread");
 };
 
+// Block metadata of one J tile: first bitmap-tier entry (a multiple of 4) and
+// first row, entries / 4 and rows.  Loaded one tile ahead of its prefetch.
+struct TileMeta {
+  int64_t d0, r0;
+  int32_t nd4, nrows;
+};
+
+// Read-only arrays read with a uniform index go through the constant address
+// space, which the compiler loads with scalar loads into scalar registers
+// (counted by the LDS/scalar counter, so waiting for them never waits for the
+// vector loads of the prefetch).
+typedef __attribute__((address_space(4))) const int64_t* const_i64_ptr;
+__device__ __forceinline__ int64_t sload_i64(const int64_t* p, int64_t i) {
+  return ((const_i64_ptr)(p))[i];
+}
+
+__device__ __forceinline__ TileMeta tile_meta(const TileSide& S, int32_t t) {
+  TileMeta m;
+  m.d0 = sload_i64(S.dbase, t);
+  m.r0 = sload_i64(S.tstart, t);
+  m.nd4 = (int32_t)((sload_i64(S.dbase, t + 1) - m.d0) >> 2);
+  m.nrows = (int32_t)(sload_i64(S.tstart, t + 1) - m.r0);
+  return m;
+}
+
 template <int DM>
 struct Prefetch {
   uint2 seg;                      // four 16-bit segment starts
   uint32_t rows[Sizes<DM>::QPT];
   float4 vals[Sizes<DM>::QPT];
   uint4 dense;                    // one 16-byte chunk of the tile's head records
-  uint32_t aux;                   // one row value: norm, head norm, length or head-part norm
-  int32_t nd4, nrows;
+  uint32_t aux;                   // one row value: norm, head norm or head-part norm
+  uint32_t rp0, rp1;              // low words of row_ptr[r], row_ptr[r+1] (row length)
+  TileMeta m;
 };
 
+// Prefetch of tile t into registers.  Every load is unconditional, from an
+// index clamped to a valid element when the lane has nothing to fetch, so no
+// loaded value is merged with a default before its use (such merges made the
+// compiler wait for the loads on the spot); validity is applied when the
+// registers are written to LDS.
 template <int DM>
-__device__ __forceinline__ void prefetch_tile(const TileSide& S, int32_t t, int tid,
-                                              Prefetch<DM>& pf) {
-  const int64_t d0 = S.dbase[t];
-  pf.nd4 = (int)((S.dbase[t + 1] - d0) >> 2);
-  const int64_t r0 = S.tstart[t];
-  pf.nrows = (int)(S.tstart[t + 1] - r0);
-  if (4 * tid < S.kds) pf.seg = reinterpret_cast<const uint2*>(S.dseg + (int64_t)t * S.kds)[tid];
+__device__ __forceinline__ void prefetch_tile(const TileSide& S, const TileMeta& m, int32_t t,
+                                              int tid, Prefetch<DM>& pf) {
+  pf.m = m;
+  pf.seg = reinterpret_cast<const uint2*>(S.dseg + (int64_t)t * S.kds)[4 * tid < S.kds ? tid : 0];
+  const int64_t q0 = m.d0 >> 2;
 #pragma unroll
   for (int i = 0; i < Sizes<DM>::QPT; ++i) {
     const int k = tid + i * SCAN_THREADS;
-    if (k < pf.nd4) {
-      pf.rows[i] = reinterpret_cast<const uint32_t*>(S.drow + d0)[k];
-      pf.vals[i] = reinterpret_cast<const float4*>(S.dval + d0)[k];
-    }
+    const int64_t q = k < m.nd4 ? q0 + k : 0;
+    pf.rows[i] = reinterpret_cast<const uint32_t*>(S.drow)[q];
+    pf.vals[i] = reinterpret_cast<const float4*>(S.dval)[q];
   }
   if constexpr (DM != DENSE_OFF) {  // thread -> (row tid / 16, chunk tid % 16)
     const int row = tid >> 4;
-    pf.dense = row < pf.nrows ? S.dense[(r0 + row) * 16 + (tid & 15)] : make_uint4(0, 0, 0, 0);
+    pf.dense = S.dense[(m.r0 + (row < m.nrows ? row : 0)) * 16 + (tid & 15)];
   }
-  // per-row values, one per thread of the first four waves
-  {
-    const int row = tid & 63, which = tid >> 6;
-    uint32_t v = 0;
-    if (row < pf.nrows) {
-      const int64_t r = r0 + row;
-      if (which == 0) v = __float_as_uint(S.norm[r]);
-      else if (which == 1) v = __float_as_uint(S.hnorm[r]);
-      else if (which == 2) v = (uint32_t)(S.rp[r + 1] - S.rp[r]);
-      else if (which == 3 && DM == DENSE_BF16) v = __float_as_uint(S.dnorm[r]);
-    }
-    pf.aux = v;
-  }
+  // per-row values, one per thread of the first four waves (the pointer is
+  // uniform per wave)
+  const int row = tid & 63, which = tid >> 6;
+  const int64_t r = m.r0 + (row < m.nrows ? row : 0);
+  const float* src = which == 1 ? S.hnorm : (which == 3 && DM == DENSE_BF16) ? S.dnorm : S.norm;
+  pf.aux = __float_as_uint(src[r]);
+  pf.rp0 = reinterpret_cast<const uint32_t*>(S.rp + r)[0];
+  pf.rp1 = reinterpret_cast<const uint32_t*>(S.rp + r + 1)[0];
 }
 
 // Workgroup barrier for hand-offs through LDS only: it waits for this wave's
@@ -181,7 +204,7 @@ __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int
 // receives the J tile of that position (INT32_MAX at the list end).
 __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
                                             const int2* __restrict__ ent, int cur, int end,
-                                            int32_t cs, int32_t ce, int rowc, float x,
+                                            int32_t cs, int32_t ce, int il, float x,
                                             uint32_t* bfill, int2* __restrict__ cprod,
                                             unsigned int& n_prod, int32_t* t_next) {
   for (;;) {
@@ -203,7 +226,7 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
       go = go && t < ce;
       if (go) {
         const uint32_t slot = atomicAdd(&bfill[t - cs], 1u);
-        cprod[slot] = make_int2(rowc + (int)(pt[q] & 0xffu), (int)(x * __int_as_float(e[q].y)));
+        cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * __int_as_float(e[q].y)));
         ++used;
       } else if (was) {
         stop_t = pt[q] == 0xffffffffu ? INT32_MAX : t;
@@ -223,7 +246,7 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
 template <int DM>
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   using Z = Sizes<DM>;
-  __shared__ int32_t tile[BAND_ROWS * TSTRIDE];     // fixed-point scores (x 2^S)
+  __shared__ __align__(16) int32_t tile[TILE_ROWS * TSTRIDE];  // fixed-point scores (x 2^S), transposed
   __shared__ __align__(8) uint16_t cJ[4 * SCAN_THREADS];  // segment starts of tile J by bitmap index
   __shared__ __align__(16) int2 eJ[Z::CJ];           // {local row, value bits}
   __shared__ uint32_t bkt[CH_TILES + 1];             // cursor-product bucket starts of the chunk
@@ -253,7 +276,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   int4* const scs = a.scs + (int64_t)blockIdx.x * a.scs_cap;
 
   // the tile is cleared cell by cell by the threshold pass, so once is enough
-  for (int c = tid0; c < BAND_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
+  for (int c = tid0; c < TILE_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
     acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
     acc_dense = 0; acc_tp = 0;
@@ -269,12 +292,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     item_s = w < (unsigned long long)a.n_items ? a.items[w] : make_int4(-1, 0, 0, 0);
   }
   __syncthreads();
-  const int4 item = item_s;
-  if (item.x < 0) break;
-  const int32_t I = item.x;
+  // uniform values: the band, its rows and the partner range live in SGPRs
+  const int32_t I = __builtin_amdgcn_readfirstlane(item_s.x);
+  if (I < 0) break;
   const int64_t i0 = a.I.tstart[I];
   const int nI = (int)(a.I.tstart[I + 1] - i0);
-  const int32_t J_first = item.y, J_end = item.z;
+  const int32_t J_first = __builtin_amdgcn_readfirstlane(item_s.y);
+  const int32_t J_end = __builtin_amdgcn_readfirstlane(item_s.z);
   // batched runs and packed queries give each band its own threshold
   const double t_band = a.band_t ? a.band_t[I] : a.t;
   int32_t q_lo = a.q_lo;
@@ -284,10 +308,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   }
 
   // ---- band set-up: this thread's band entries live in registers ---------
-  // Entry k = tid + q * SCAN_THREADS of the band block: (row * TSTRIDE,
-  // bitmap index, value * 2^S).  The block is ordered by frequency rank, so
+  // Entry k = tid + q * SCAN_THREADS of the band block: (band row, bitmap
+  // index, value * 2^S).  The block is ordered by frequency rank, so
   // the lanes of a wave hold entries of the same or similar features.
-  int32_t e_rf[Z::IPT];  // row * TSTRIDE << 12 | bitmap index (or -1)
+  int32_t e_rf[Z::IPT];  // band row << 12 | bitmap index (or -1)
   {
     const int64_t d0 = a.I.dbase[I];
     const int nE = a.I.dseg[(int64_t)I * a.I.kds + a.KD];  // entries of the band block
@@ -299,7 +323,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       // the tile segment once in the list-visit statistics)
       const int32_t f = ok ? (int32_t)a.I.dfeat[d0 + k] : 0;
       const bool seg_head = ok && (k == 0 || (int32_t)a.I.dfeat[d0 + k - 1] != f);
-      e_rf[q] = ok ? ((((int32_t)a.I.drow[d0 + k] * TSTRIDE) << 12) | f | (seg_head ? (1 << 26) : 0)) : -1;
+      e_rf[q] = ok ? ((((int32_t)a.I.drow[d0 + k]) << 12) | f | (seg_head ? (1 << 26) : 0)) : -1;
       vI[k] = ok ? a.I.dval[d0 + k] * a.qscale : 0.f;  // exact: power of 2
     }
   }
@@ -334,7 +358,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __syncthreads();
 
   Prefetch<DM> pf;
-  if (J_first < J_end) prefetch_tile<DM>(a.J, J_first, tid, pf);
+  TileMeta meta_next{};  // metadata of tile J + 1 while tile J is processed
+  if (J_first < J_end) {
+    prefetch_tile<DM>(a.J, tile_meta(a.J, J_first), J_first, tid, pf);
+    meta_next = tile_meta(a.J, J_first + 1 < J_end ? J_first + 1 : J_first);
+  }
   int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
 
   for (int32_t J = J_first; J < J_end; ++J) {
@@ -381,7 +409,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int4 r = a.I.srec[sb + k];
         int32_t tn;
         unsigned int np = 0;
-        const int cur = walk_scatter(a.S.pos, a.S.ent, st.x, st.y, ch_lo, ce, r.x * TSTRIDE,
+        const int cur = walk_scatter(a.S.pos, a.S.ent, st.x, st.y, ch_lo, ce, r.x,
                                      __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn);
         scs[k] = make_int4(cur, st.y, tn, 0);
         if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
@@ -390,42 +418,54 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       ch_hi = ce;
     }
 
-    const int64_t j0 = a.J.tstart[J];
-    const int nJ = pf.nrows;
+    const int64_t j0 = pf.m.r0;
+    const int nJ = pf.m.nrows;
 
     // ---- write the prefetched tile J into LDS --------------------------------
     if (4 * tl < a.J.kds) reinterpret_cast<uint2*>(cJ)[tl] = pf.seg;
 #pragma unroll
     for (int i = 0; i < Z::QPT; ++i) {
       const int k = tl + i * SCAN_THREADS;
-      if (k < pf.nd4) {
+      if (k < pf.m.nd4) {
         const float vv[4] = {pf.vals[i].x, pf.vals[i].y, pf.vals[i].z, pf.vals[i].w};
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu), __float_as_int(vv[q]));
+          eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu) * TSTRIDE, __float_as_int(vv[q]));
       }
     }
-    if (tl < 4 * TILE_ROWS) {
+    if (tl < 4 * TILE_ROWS) {  // rows past the tile read as zero
       const int row = tl & 63, which = tl >> 6;
-      if (which == 0) normJ[row] = __uint_as_float(pf.aux);
-      else if (which == 1) hnJ[row] = __uint_as_float(pf.aux);
-      else if (which == 2) lenJ[row] = (int32_t)pf.aux;
-      else if (DM == DENSE_BF16) dnJ[row] = __uint_as_float(pf.aux);
+      const bool ok = row < nJ;
+      const float v = ok ? __uint_as_float(pf.aux) : 0.f;
+      if (which == 0) normJ[row] = v;
+      else if (which == 1) hnJ[row] = v;
+      else if (which == 2) lenJ[row] = ok ? (int32_t)(pf.rp1 - pf.rp0) : 0;
+      else if (DM == DENSE_BF16) dnJ[row] = v;
     }
-    if constexpr (DM != DENSE_OFF) dB[dslot(tl >> 4, tl & 15)] = pf.dense;
+    if constexpr (DM != DENSE_OFF)
+      dB[dslot(tl >> 4, tl & 15)] = (tl >> 4) < nJ ? pf.dense : make_uint4(0, 0, 0, 0);
 
     lds_barrier();
 
-    // prefetch tile J+1 (registers; written to LDS at the top of the next pass)
-    if (J + 1 < J_end) prefetch_tile<DM>(a.J, J + 1, tl, pf);
-
     // this thread's first cursor product of tile J: its load completes behind
-    // the bitmap-tier products
+    // the bitmap-tier products.  It is issued before the prefetch of the next
+    // tile: loads complete in order, so waiting for it does not wait for the
+    // prefetch
     int32_t pb = 0, pe = 0;
     if (nS > 0) { pb = (int32_t)bkt[J - ch_lo]; pe = (int32_t)bkt[J - ch_lo + 1]; }
     const int32_t k0 = pb + tl;
-    int2 hp = make_int2(0, 0);
-    if (k0 < pe) hp = cprod[k0];
+    const int2 hp = cprod[k0 < pe ? k0 : 0];  // unconditional (see prefetch_tile)
+
+    // prefetch tile J+1 (registers; written to LDS at the top of the next
+    // pass) and the metadata of tile J+2.  Both are unconditional (the last
+    // tile is fetched again at the end of the range), so the code between
+    // here and the uses is straight-line and the compiler waits for exactly
+    // the loads it needs.
+    {
+      const int32_t t1 = J + 1 < J_end ? J + 1 : J;
+      prefetch_tile<DM>(a.J, meta_next, t1, tl, pf);
+      meta_next = tile_meta(a.J, J + 2 < J_end ? J + 2 : J_end - 1);
+    }
 
     // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
     // Two passes of half the entries: the LDS lookups of a pass are
@@ -446,7 +486,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int q = 0; q < Z::IPT / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
         my_visits += 1u + (((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u);
-        const int ib = (e_rf[h + q] >> 12) & 0x3fff;
+        const int ib = (e_rf[h + q] >> 12) & 0x3fff;  // band row
         const float u = vI[tl + (h + q) * SCAN_THREADS];
         // four entries per step: their LDS reads issue together, then the atomics
         int y = y0[q];
@@ -499,26 +539,31 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         if constexpr (DM == DENSE_I8) {
           // limb products grouped by weight 2^(8*(6-w)); groups w <= 4 are
           // kept (the dropped w = 5, 6 are inside eps_dense)
+          // hi = (c0 << 16) + (c1 << 8) + c2 (weight 2^32) is folded group by
+          // group (shifts distribute over wrapping addition); lo = (c3 << 8) +
+          // c4 (weight 2^16) is truncated by a right shift, so it is formed whole
           const v4i z = {0, 0, 0, 0};
-          v4i c0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[0], z, 0, 0, 0);
+          const v4i c0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[0], z, 0, 0, 0);
           v4i c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[1], z, 0, 0, 0);
-          v4i c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[2], z, 0, 0, 0);
-          v4i c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[3], z, 0, 0, 0);
-          v4i c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[3], z, 0, 0, 0);
           c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[0], c1, 0, 0, 0);
-          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[1], c2, 0, 0, 0);
-          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[2], c3, 0, 0, 0);
-          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[2], c4, 0, 0, 0);
-          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[0], c2, 0, 0, 0);
-          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[1], c3, 0, 0, 0);
-          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[1], c4, 0, 0, 0);
-          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[0], c3, 0, 0, 0);
+          int32_t hv[4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int32_t hi = (c0[v] << 16) + (c1[v] << 8) + c2[v];  // weight 2^32
-            const int32_t lo = (c3[v] << 8) + c4[v];                   // weight 2^16
-            dv[4 * blk + v] = (hi << a.dsh_hi) + (lo >> a.dsh_lo);
-          }
+          for (int v = 0; v < 4; ++v) hv[v] = (c0[v] << 16) + (c1[v] << 8);
+          v4i c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[2], z, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[1], c2, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[0], c2, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) hv[v] = (hv[v] + c2[v]) << a.dsh_hi;
+          v4i c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[0], bv[3], z, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[2], c3, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[1], c3, 0, 0, 0);
+          c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[0], c3, 0, 0, 0);
+          v4i c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[1], bv[3], z, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[2], bv[2], c4, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[3], bv[1], c4, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            dv[4 * blk + v] = hv[v] + (((c3[v] << 8) + c4[v]) >> a.dsh_lo);
         } else {
           v4f c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -542,16 +587,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if constexpr (DM != DENSE_OFF) {  // the filter needs the dense part in the tile
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-          tile[(il_b + (m & 3)) * TSTRIDE + jl_b + 16 * (m >> 2)] += dv[m];
+          tile[(jl_b + 16 * (m >> 2)) * TSTRIDE + il_b + (m & 3)] += dv[m];
           dv[m] = 0;
         }
         lds_barrier();
       }
 #pragma unroll 1
       for (int c = tl; c < BAND_ROWS * TILE_ROWS; c += SCAN_THREADS) {
-        const int il = c / TILE_ROWS, jl = c % TILE_ROWS;
+        const int jl = c / BAND_ROWS, il = c % BAND_ROWS;
         if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
-        int32_t* cell = &tile[il * TSTRIDE + jl];
+        int32_t* cell = &tile[jl * TSTRIDE + il];
         const int32_t part = *cell;
         // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
         const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jl];
@@ -603,12 +648,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const bool plain = q_lo > 0 && (!a.self || dj >= BAND_ROWS);
     int32_t qmax = INT32_MIN;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      int32_t* cell = &tile[(il_b + (m & 3)) * TSTRIDE + jl_b + 16 * (m >> 2)];
-      dv[m] += *cell;  // full fixed-point score of the cell
-      *cell = 0;
-      qmax = dv[m] > qmax ? dv[m] : qmax;
+    for (int blk = 0; blk < 2; ++blk) {  // four band rows of one tile row: one 16-byte word
+      int4* w4 = reinterpret_cast<int4*>(&tile[(jl_b + 16 * blk) * TSTRIDE + il_b]);
+      const int4 c4 = *w4;
+      *w4 = make_int4(0, 0, 0, 0);
+      dv[4 * blk + 0] += c4.x;  // full fixed-point score of the cell
+      dv[4 * blk + 1] += c4.y;
+      dv[4 * blk + 2] += c4.z;
+      dv[4 * blk + 3] += c4.w;
     }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) qmax = dv[m] > qmax ? dv[m] : qmax;
     unsigned cmask = 0;
     if (__ballot(!plain || qmax >= q_lo)) {
       // valid rows of this thread's two blocks: v < lim[blk] (v = m % 4)
@@ -641,7 +691,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
         if (c && pos < (unsigned)Z::CJ) cand[pos] = make_int2((il << 8) | jl, dv[m]);
         if (c && pos >= (unsigned)Z::CJ) {  // list full: the cell keeps its score
-          tile[il * TSTRIDE + jl] = dv[m];
+          tile[jl * TSTRIDE + il] = dv[m];
           ovf |= 1u << m;
         }
       }
@@ -668,7 +718,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         il = il_b + (m & 3);
         jl = jl_b + 16 * (m >> 2);
         if ((ovf >> m) & 1u) {
-          int32_t* cell = &tile[il * TSTRIDE + jl];
+          int32_t* cell = &tile[jl * TSTRIDE + il];
           q = *cell;
           *cell = 0;
           is_cand = true;

@@ -286,6 +286,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip", default="", help="comma list of legs to skip: " + ",".join(ALL_LEGS))
     ap.add_argument("--only-main", action="store_true", help="skip every leg beside the headline steps")
+    ap.add_argument("--copy-check-mib", type=int, default=0,
+                    help="before the steps, copy a buffer of this many MiB device to device three "
+                         "times (a kernel of known traffic that scales HBM counters)")
     ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--serving-rows", type=int, default=2_000_000)
     ap.add_argument("--serving-budget", type=int, default=64 << 30)
@@ -307,6 +310,15 @@ def main():
     log(f"generated large workload n={n} nnz={nnz} in {time.time() - t0:.1f}s")
 
     hip = apss.Hip()
+    if args.copy_check_mib > 0:
+        nb = args.copy_check_mib << 20
+        a, b = hip.malloc(nb), hip.malloc(nb)
+        for _ in range(3):
+            hip.d2d(b, a, nb)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        log(f"copy check: 3 x {args.copy_check_mib} MiB device-to-device")
     X = apss.DeviceCSR(hip, rp, col, val, d)
     stream = hip.stream()
     e0, e1 = hip.event(), hip.event()
