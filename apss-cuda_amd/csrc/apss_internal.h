@@ -158,8 +158,9 @@ struct ScanArgs {
   int32_t rescore_all;
   int32_t n_items;           // work items of this launch
   const int4* items;         // {band I, first J tile, one past the last J tile, unused}
-  unsigned long long* stats; // [0] products [1] head pairs [2] cells [3] list visits
-                             // [4] cursor products [5] dense records fetched [6] tile pairs
+  unsigned long long* stats; // [0] products [1] head pairs [3] list visits [4] cursor products
+                             // (cells, tile pairs and dense records follow from the work
+                             // items and are counted on the host)
   const double* band_t;      // per band: threshold (NULL: t)
   double eps_max;            // largest cell error bound (for per-band fast rejects)
   int4* scs;                 // per cursor record of the current band: {position, end,

@@ -639,6 +639,16 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     je[b] = e;
     partner_tiles += std::max(0, e - f);
   }
+  // Pair cells, dense head rows and tile pairs of the launch, from the work
+  // items: cells of rows [i0, i1) x [j0, j1), only i < j for a self-join.
+  auto cells_of = [&](int64_t i0, int64_t i1, int64_t j0, int64_t j1) -> int64_t {
+    if (i1 <= i0 || j1 <= j0) return 0;
+    if (!rc.self) return (i1 - i0) * (j1 - j0);
+    int64_t c = std::max<int64_t>(0, std::min(i1, j0) - i0) * (j1 - j0);  // rows below j0
+    const int64_t lo = std::max(i0, j0), hi = std::min(i1, j1) - 1;         // rows inside
+    if (hi >= lo) c += (hi - lo + 1) * (j1 - 1) - (lo + hi) * (hi - lo + 1) / 2;
+    return c;
+  };
   auto cut_items = [&](int32_t chunk, int32_t stride, std::vector<int4>& v) {
     int64_t tp = 0;
     for (int32_t b = 0; b < bands.T; b += stride)
@@ -654,6 +664,13 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   std::vector<int4> hitems;
   cut_items(chunk, 1, hitems);
   const int32_t n_items = (int32_t)hitems.size();
+  int64_t host_cells = 0, host_dense = 0;
+  for (const int4& it : hitems) {
+    const int64_t i0 = bands.cuts[it.x], i1 = bands.cuts[it.x + 1];
+    const int64_t j0 = tiles.cuts[it.y], j1 = tiles.cuts[it.z];
+    host_cells += cells_of(i0, i1, j0, j1);
+    if (tr.dm != DENSE_OFF) host_dense += (j1 - j0) + (i1 - i0);
+  }
   int4* items = ar.alloc<int4>(std::max(n_items, 1));
   if (n_items > 0)
     APSS_HIP(hipMemcpyAsync(items, hitems.data(), sizeof(int4) * n_items, hipMemcpyHostToDevice, st));
@@ -783,11 +800,11 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     }
     so.products = (int64_t)h[2];
     so.head_pairs = (int64_t)h[3];
-    so.cells = (int64_t)h[4];
+    so.cells = host_cells;
     so.visits = (int64_t)h[5];
     so.cursor = (int64_t)h[6];
-    so.dense_rows = (int64_t)h[7];
-    so.tile_pairs = (int64_t)h[8];
+    so.dense_rows = host_dense;
+    so.tile_pairs = partner_tiles;
     so.band = nband;
     so.n_dense = tr.n_dense;
     if (nout + nband > cap) {  // room for every band pair before the rescore

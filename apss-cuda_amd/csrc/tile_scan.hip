@@ -179,7 +179,10 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, int tl, uint32_t* w
 // Cursor walks over one record's list from position cur, visiting the entries
 // whose J tile lies below ce (lists are row-sorted, so those come first).
 // Positions are read eight at a time so one memory round trip covers up to
-// eight entries.
+// eight entries: the eight loads are unconditional, from positions clamped to
+// the list's last entry (callers guarantee cur < end), and entries past the
+// end are masked afterwards, so the loads issue together and are waited for
+// once.
 constexpr int WALK_BATCH = 8;
 
 __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
@@ -187,11 +190,11 @@ __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int
   for (;;) {
     uint32_t pt[WALK_BATCH];
 #pragma unroll
-    for (int q = 0; q < WALK_BATCH; ++q) pt[q] = cur + q < end ? pos[cur + q] : 0xffffffffu;
+    for (int q = 0; q < WALK_BATCH; ++q) pt[q] = pos[cur + q < end ? cur + q : end - 1];
     bool go = true;
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
-      const int32_t t = (int32_t)(pt[q] >> 8);
+      const int32_t t = cur + q < end ? (int32_t)(pt[q] >> 8) : INT32_MAX;
       go = go && t < ce;
       if (go) atomicAdd(&bkt[t - cs], 1u);
     }
@@ -212,16 +215,17 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
     int2 e[WALK_BATCH];
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
-      const bool ok = cur + q < end;
-      pt[q] = ok ? pos[cur + q] : 0xffffffffu;
-      e[q] = ok ? ent[cur + q] : make_int2(0, 0);
+      const int c = cur + q < end ? cur + q : end - 1;
+      pt[q] = pos[c];
+      e[q] = ent[c];
     }
     bool go = true;
     int used = 0;
     int32_t stop_t = INT32_MAX;
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
-      const int32_t t = (int32_t)(pt[q] >> 8);
+      const bool in_list = cur + q < end;
+      const int32_t t = in_list ? (int32_t)(pt[q] >> 8) : INT32_MAX;
       const bool was = go;
       go = go && t < ce;
       if (go) {
@@ -229,7 +233,7 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
         cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * __int_as_float(e[q].y)));
         ++used;
       } else if (was) {
-        stop_t = pt[q] == 0xffffffffu ? INT32_MAX : t;
+        stop_t = t;
       }
     }
     cur += used;
@@ -265,8 +269,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ int32_t ch_end_s;
   __shared__ unsigned int cand_n;                    // candidate cells of the tile pair
   __shared__ unsigned int rec_next;                  // cursor records handed out so far
-  __shared__ unsigned long long acc_products, acc_head, acc_cells, acc_visits, acc_cursor;
-  __shared__ unsigned long long acc_dense, acc_tp;
+  __shared__ unsigned long long acc_products, acc_head, acc_visits, acc_cursor;
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
   static_assert(BAND_ROWS == 128 && TILE_ROWS == 64 && SCAN_THREADS == 1024,
                 "MFMA layout: 16 waves x two 16x16 blocks cover the 128 x 64 tile");
@@ -278,8 +281,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // the tile is cleared cell by cell by the threshold pass, so once is enough
   for (int c = tid0; c < TILE_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
-    acc_products = 0; acc_head = 0; acc_cells = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
-    acc_dense = 0; acc_tp = 0;
+    acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
   }
   unsigned int my_products = 0, my_visits = 0;
 
@@ -625,23 +627,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // list capacity stay in their cells and are settled by their own thread.
     int2* const cand = eJ;
     const int64_t dj = j0 - i0;
-    if (tl == 0) {  // valid cells of the tile pair (closed form of the per-column limits)
-      int64_t c;
-      if (!a.self) {
-        c = (int64_t)nI * nJ;
-      } else {  // sum over jl < nJ of clamp(dj + jl, 0, nI)
-        auto G = [nI](int64_t y) -> int64_t {  // sum of clamp(x, 0, nI) over 0 <= x < y
-          if (y <= 0) return 0;
-          if (y <= (int64_t)nI + 1) return y * (y - 1) / 2;
-          return (int64_t)nI * (nI + 1) / 2 + (y - nI - 1) * nI;
-        };
-        c = G(dj + nJ) - G(dj);
-      }
-      acc_cells += (unsigned long long)c;
-      acc_tp += 1;
-      // head records: the tile's rows per tile pair, the band's once per item
-      if (DM != DENSE_OFF) acc_dense += (unsigned long long)(nJ + (J == J_first ? nI : 0));
-    }
     // Cells past the band or tile height receive no products and a zero head,
     // so with q_lo > 0 only the self-join diagonal needs an explicit validity
     // test: off the diagonal one max over the eight cells rejects them all.
@@ -795,11 +780,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   if (tid0 == 0) {
     atomicAdd(&a.stats[0], acc_products + acc_cursor);
     atomicAdd(&a.stats[1], acc_head);
-    atomicAdd(&a.stats[2], acc_cells);
     atomicAdd(&a.stats[3], acc_visits);
     atomicAdd(&a.stats[4], acc_cursor);
-    atomicAdd(&a.stats[5], acc_dense);
-    atomicAdd(&a.stats[6], acc_tp);
   }
 }
 
