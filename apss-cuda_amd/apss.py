@@ -458,13 +458,19 @@ class Server:
         pairs, _, lat = self.wait_info(ticket)
         return pairs, lat
 
-    def wait_info(self, ticket):
-        """(pairs, counters of the launch that served the request, latency ms)."""
+    def wait_info(self, ticket, keep_pairs=True):
+        """(pairs, counters of the launch that served the request, latency ms);
+        with keep_pairs=False the pairs are released uncopied (pairs = count)."""
         r = Result()
         lat = c_double()
         rc = self.L.apss_server_wait(self.h, int(ticket), ctypes.byref(r), ctypes.byref(lat))
         _check(rc)
-        pairs, info = _take_result(r)
+        if keep_pairs:
+            pairs, info = _take_result(r)
+        else:
+            info = {f: getattr(r, f) for f, _ in Result._fields_ if f not in ("pairs", "internal_")}
+            pairs = r.count
+            lib().apss_result_free(ctypes.byref(r))
         return pairs, info, lat.value
 
     def cancel(self, ticket):

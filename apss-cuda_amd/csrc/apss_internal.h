@@ -161,7 +161,8 @@ struct ScanArgs {
   unsigned long long* stats; // [0] products [1] head pairs [3] list visits [4] cursor products
                              // (cells, tile pairs and dense records follow from the work
                              // items and are counted on the host)
-  const double* band_t;      // per band: threshold (NULL: t)
+  const double* band_t;      // per band: lowest threshold of its rows (NULL: t)
+  const double* row_t;       // per I row: threshold (NULL: the band's)
   double eps_max;            // largest cell error bound (for per-band fast rejects)
   int4* scs;                 // per cursor record of the current band: {position, end,
                              // J tile at position (INT32_MAX: exhausted), unused};

@@ -38,9 +38,9 @@ template <int DM>
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
-                          const int32_t* colJ, const float* valJ, double t, int64_t i_off,
-                          int64_t j_off, apss_pair* out, unsigned long long* out_cnt,
-                          int64_t out_cap);
+                          const int32_t* colJ, const float* valJ, double t, const double* row_t,
+                          int64_t i_off, int64_t j_off, apss_pair* out,
+                          unsigned long long* out_cnt, int64_t out_cap);
 
 // ---- device memory --------------------------------------------------------
 
@@ -171,7 +171,9 @@ struct RunConfig {
   int32_t kd_cap = 0;
   int64_t i_off = 0, j_off = 0;
   std::vector<int32_t> band_jend;      // per band: one past its last partner tile (empty: all)
-  const double* band_t = nullptr;      // device, per band (batched runs, packed queries)
+  const double* band_t = nullptr;      // device, per band: lowest row threshold (batched runs,
+                                       // packed queries)
+  const double* row_t = nullptr;       // device, per I row: threshold (NULL: t)
   double t_min = 0.0;                  // smallest per-band threshold
   int64_t out_cap_hint = 0;            // initial output capacity in pairs (0: from free memory)
 };

@@ -561,7 +561,8 @@ void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
   s.dnorm = nullptr;
   if (tr.dm == DENSE_OFF) return;
   // int8 head: |x| <= maxnorm, so x * 2^B stays within 2^28 (four limbs)
-  const double m = std::max((double)s.maxnorm, 1e-30);
+  // (a side whose values are all zero is exact at any scale: use norm 1)
+  const double m = s.maxnorm > 0.f ? (double)s.maxnorm : 1.0;
   s.dense_B = 28 - (int)std::ceil(std::log2(m));
   s.dense = ar.alloc<uint4>(std::max<int64_t>(s.n, 1) * 16);
   if (tr.dm == DENSE_BF16) s.dnorm = ar.alloc<float>(s.n);
@@ -579,8 +580,12 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
                             const Side& J, const TileSet& tiles, const Tiers& tr,
                             const RunConfig& rc) {
   ScanOutput so;
+  so.n_dense = tr.n_dense;
+  if (I.n == 0 || J.n == 0 || bands.T == 0 || tiles.T == 0) return so;  // no pair cells
   const int64_t maxlen = std::max(I.maxlen, J.maxlen);
-  const double mI = std::max((double)I.maxnorm, 1e-30), mJ = std::max((double)J.maxnorm, 1e-30);
+  // (a side whose values are all zero is exact at any scale: use norm 1)
+  const double mI = I.maxnorm > 0.f ? (double)I.maxnorm : 1.0;
+  const double mJ = J.maxnorm > 0.f ? (double)J.maxnorm : 1.0;
   // fixed-point scale 2^S with 2^S * max|x||y| <= 2^30 (partial sums < 2^31)
   int S = 30 - (int)std::ceil(std::log2(mI * mJ));
   S = std::max(-100, std::min(S, 100));
@@ -707,6 +712,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.band_cnt = ctr + 1;
   a.rescore_all = rc.rescore_all;
   a.band_t = rc.band_t;
+  a.row_t = rc.row_t;
   a.eps_max = eps_max;
   a.stats = ctr + 2;
   a.scs = scs;
@@ -817,7 +823,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     tm.start();
     if (nband > 0) {
       hipLaunchKernelGGL(k_rescore, dim3(blocks_for(nband, 256, 1 << 30)), dim3(256), 0, st, band,
-                         nband, I.rp, I.col, I.val, J.rp, J.col, J.val, rc.t, rc.i_off, rc.j_off,
+                         nband, I.rp, I.col, I.val, J.rp, J.col, J.val, rc.t, rc.row_t, rc.i_off, rc.j_off,
                          out, ctr + 0, cap);
       APSS_HIP(hipGetLastError());
     }
@@ -1037,6 +1043,14 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   double* dt = ar.alloc<double>(bands.T);
   if (bands.T > 0)
     APSS_HIP(hipMemcpyAsync(dt, ht.data(), sizeof(double) * bands.T, hipMemcpyHostToDevice, st));
+  // per row: its instance's threshold (the band kernel and the FP64 rescore
+  // decide every pair against the threshold of its own instance)
+  std::vector<double> hrt((size_t)std::max<int64_t>(n_all, 1));
+  for (int32_t k = 0; k < count; ++k)
+    for (int64_t r = off[k]; r < off[k + 1]; ++r) hrt[(size_t)r] = t[k];
+  double* drt = ar.alloc<double>(n_all);
+  if (n_all > 0)
+    APSS_HIP(hipMemcpyAsync(drt, hrt.data(), sizeof(double) * n_all, hipMemcpyHostToDevice, st));
   const double ms_index = tm.stop_ms();
   RunConfig rc;
   rc.t = t_min;
@@ -1045,6 +1059,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   rc.rescore_all = o->rescore_all;
   rc.band_jend = std::move(hjend);
   rc.band_t = dt;
+  rc.row_t = drt;
   rc.t_min = t_min;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   // split the pair list by instance

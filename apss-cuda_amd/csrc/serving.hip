@@ -125,12 +125,43 @@ static std::unique_ptr<Segment> build_segment(HostCSR&& h, int32_t p, const apss
   return sg;
 }
 
+// Growable host pair buffer (malloc, no zero fill); release() hands the block
+// to a result, which apss_result_free releases with free().
+struct PairBuf {
+  apss_pair* p = nullptr;
+  int64_t n = 0, cap = 0;
+  PairBuf() = default;
+  PairBuf(const PairBuf&) = delete;
+  PairBuf& operator=(const PairBuf&) = delete;
+  ~PairBuf() { std::free(p); }
+  apss_pair* grow(int64_t extra) {  // room for `extra` more pairs; returns where they go
+    if (n + extra > cap) {
+      const int64_t c = std::max<int64_t>(n + extra, std::max<int64_t>(1, cap + cap / 2));
+      void* q = std::realloc(p, sizeof(apss_pair) * (size_t)c);
+      if (!q) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+      p = (apss_pair*)q;
+      cap = c;
+    }
+    apss_pair* at = p + n;
+    n += extra;
+    return at;
+  }
+  apss_pair* release() {
+    if (!p) p = (apss_pair*)std::malloc(sizeof(apss_pair));
+    if (!p) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+    apss_pair* q = p;
+    p = nullptr;
+    n = cap = 0;
+    return q;
+  }
+};
+
 // Query rows (host CSR) against every segment; `bounds` are the first rows of
 // the requests packed into Q (ascending, excluding 0) and t_req their
 // thresholds (one per request).  Pairs (query row, corpus row) go to `pairs`.
 static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int64_t>& bounds,
                         const std::vector<double>& t_req, hipStream_t st, int64_t budget,
-                        int rescore_all, std::vector<apss_pair>& pairs, apss_result* stats,
+                        int rescore_all, PairBuf& pairs, apss_result* stats,
                         DevicePool* pool) {
   std::shared_lock<std::shared_mutex> lk(ix.mu);
   if (Q.d > ix.d) throw Error{APSS_E_INVALID, "query has more columns than the corpus"};
@@ -141,21 +172,30 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
   tm.start();
   Side qs = prepare_side(ar, st, qv, ix.p);
   const double ms_up = tm.stop_ms();
+  // Every query row carries its request's threshold, so requests packed in
+  // one launch share bands (a band may hold rows of several requests).
   std::vector<int64_t> starts(1, 0);
   starts.insert(starts.end(), bounds.begin(), bounds.end());
   double t_min = INFINITY;
   for (double x : t_req) t_min = std::min(t_min, x);
+  std::vector<double> hrt((size_t)std::max<int64_t>(Q.n(), 1), t_min);
+  for (size_t k = 0; k < starts.size() && k < t_req.size(); ++k) {
+    const int64_t e = k + 1 < starts.size() ? starts[k + 1] : Q.n();
+    for (int64_t r = starts[k]; r < e; ++r) hrt[(size_t)r] = t_req[k];
+  }
+  double* drt = ar.alloc<double>(Q.n());
+  if (Q.n() > 0)
+    APSS_HIP(hipMemcpyAsync(drt, hrt.data(), sizeof(double) * Q.n(), hipMemcpyHostToDevice, st));
   for (const auto& sg : ix.segs) {
     tm.start();
-    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, cap_i_for(sg->tr.dm), CAP_S, true,
-                                bounds);
+    TileSet bands = build_tiles(ar, st, qs, sg->tr, 0, BAND_ROWS, cap_i_for(sg->tr.dm), CAP_S, true, {});
     Side qd = qs;  // the query's head records follow this segment's head features
     build_dense(ar, st, qd, sg->tr);
     std::vector<double> ht((size_t)std::max(bands.T, 1), t_min);
-    for (int32_t b = 0; b < bands.T; ++b) {
-      const int64_t r = bands.cuts[b];
-      const size_t k = (size_t)(std::upper_bound(starts.begin(), starts.end(), r) - starts.begin()) - 1;
-      ht[b] = t_req[k];
+    for (int32_t b = 0; b < bands.T; ++b) {  // the lowest threshold of the band's rows
+      double m = INFINITY;
+      for (int64_t r = bands.cuts[b]; r < bands.cuts[b + 1]; ++r) m = std::min(m, hrt[(size_t)r]);
+      ht[b] = m;
     }
     double* dt = ar.alloc<double>(bands.T);
     if (bands.T > 0)
@@ -168,14 +208,13 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     rc.rescore_all = rescore_all;
     rc.j_off = sg->row_off;
     rc.band_t = dt;
+    rc.row_t = drt;
     rc.t_min = t_min;
     rc.out_cap_hint = query_out_hint(Q.n(), sg->side.n);
     ScanOutput so = scan_and_rescore(ar, st, qd, bands, sg->side, sg->tiles, sg->tr, rc);
-    const size_t at = pairs.size();
-    pairs.resize(at + (size_t)so.count);
+    apss_pair* at = pairs.grow(so.count);
     if (so.count)
-      APSS_HIP(hipMemcpyAsync(pairs.data() + at, so.pairs, sizeof(apss_pair) * so.count,
-                              hipMemcpyDeviceToHost, st));
+      APSS_HIP(hipMemcpyAsync(at, so.pairs, sizeof(apss_pair) * so.count, hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     if (stats) {
       stats->cells += so.cells;
@@ -274,37 +313,70 @@ static void worker_loop(Server* s, int slot) {
       s->in_use += est;
     }
     // one launch sequence for the packed batch
+    // (requests without rows take no part and complete with no pairs)
     HostCSR Q;
     Q.d = s->idx->d;
-    std::vector<int64_t> bounds;
+    std::vector<int64_t> bounds, lstart;  // interior request starts; start of each live request
     std::vector<double> ts;
-    for (Request* r : batch) {
+    std::vector<size_t> live;             // batch positions of the requests with rows
+    for (size_t k = 0; k < batch.size(); ++k) {
+      Request* r = batch[k];
+      if (r->q.n() == 0) continue;
       if (Q.n() > 0) bounds.push_back(Q.n());
+      lstart.push_back(Q.n());
       Q.append(r->q.view());
       ts.push_back(r->t);
+      live.push_back(k);
     }
-    std::vector<apss_pair> pairs;
+    PairBuf pairs;
     apss_result stats{};
     int rc = APSS_OK;
     std::string err;
     try {
       if (!st) throw Error{APSS_E_DEVICE, "stream creation failed"};
-      query_index(*s->idx, Q, bounds, ts, st, est, 0, pairs, &stats, &pool);
+      if (!live.empty()) query_index(*s->idx, Q, bounds, ts, st, est, 0, pairs, &stats, &pool);
     } catch (const Error& e) {
       rc = e.code;
       err = e.what;
     } catch (...) {
       rc = APSS_E_DEVICE;
     }
-    // split by request (pairs carry packed query row ids)
-    std::vector<int64_t> starts(1, 0);
-    starts.insert(starts.end(), bounds.begin(), bounds.end());
-    std::vector<std::vector<apss_pair>> per(batch.size());
-    for (const apss_pair& pr : pairs) {
-      const size_t k = (size_t)(std::upper_bound(starts.begin(), starts.end(), (int64_t)pr.i) - starts.begin()) - 1;
-      apss_pair x = pr;
-      x.i -= (uint32_t)starts[k];
-      per[k].push_back(x);
+    // split by request: pairs carry packed query rows; one request takes the
+    // whole buffer, several are split by a counting pass
+    std::vector<apss_pair*> outp(batch.size(), nullptr);
+    std::vector<int64_t> outn(batch.size(), 0);
+    try {
+      if (rc == APSS_OK && live.size() == 1) {
+        outn[live[0]] = pairs.n;
+        outp[live[0]] = pairs.release();
+      } else if (rc == APSS_OK && live.size() > 1) {
+        std::vector<int32_t> req_of((size_t)Q.n());  // live index of every packed row
+        for (size_t m = 0; m < live.size(); ++m) {
+          const int64_t e = m + 1 < live.size() ? lstart[m + 1] : Q.n();
+          for (int64_t r = lstart[m]; r < e; ++r) req_of[(size_t)r] = (int32_t)m;
+        }
+        std::vector<int64_t> cnt(live.size(), 0), fill(live.size(), 0);
+        for (int64_t q = 0; q < pairs.n; ++q) ++cnt[(size_t)req_of[pairs.p[q].i]];
+        for (size_t m = 0; m < live.size(); ++m) {
+          outn[live[m]] = cnt[m];
+          outp[live[m]] = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(cnt[m], 1));
+          if (!outp[live[m]]) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+        }
+        for (int64_t q = 0; q < pairs.n; ++q) {
+          apss_pair x = pairs.p[q];
+          const size_t m = (size_t)req_of[x.i];
+          x.i -= (uint32_t)lstart[m];
+          outp[live[m]][fill[m]++] = x;
+        }
+      }
+      for (size_t k = 0; k < batch.size(); ++k)  // requests without rows (or pairs)
+        if (rc == APSS_OK && !outp[k]) {
+          outp[k] = (apss_pair*)std::malloc(sizeof(apss_pair));
+          if (!outp[k]) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+        }
+    } catch (const Error& e) {
+      rc = e.code;
+      err = e.what;
     }
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -313,11 +385,9 @@ static void worker_loop(Server* s, int slot) {
         Request* r = batch[k];
         r->rc = rc;
         r->res = stats;
-        r->res.count = (int64_t)per[k].size();
-        r->res.pairs = (apss_pair*)std::malloc(sizeof(apss_pair) * std::max<size_t>(per[k].size(), 1));
-        if (r->res.pairs && !per[k].empty())
-          std::memcpy(r->res.pairs, per[k].data(), sizeof(apss_pair) * per[k].size());
-        if (!r->res.pairs) r->rc = APSS_E_NOMEM;
+        r->res.count = rc == APSS_OK ? outn[k] : 0;
+        r->res.pairs = rc == APSS_OK ? outp[k] : nullptr;
+        if (rc != APSS_OK && outp[k]) std::free(outp[k]);
         r->res.on_device = 0;
         r->res.p_used = s->idx->p;
         r->state = Request::DONE;
@@ -434,13 +504,11 @@ int apss_index_query(apss_index* ix, const apss_csr* queries, double t, apss_res
     auto t0 = std::chrono::steady_clock::now();
     HostCSR q = to_host(*queries, I.device);
     StreamScope ss(I.opts.stream, I.device);
-    std::vector<apss_pair> pairs;
+    PairBuf pairs;
     query_index(I, q, {}, {t}, ss.s, I.opts.mem_budget, I.opts.rescore_all, pairs, out, nullptr);
-    if (I.opts.sort_output) sort_pairs(pairs.data(), (int64_t)pairs.size());
-    out->count = (int64_t)pairs.size();
-    out->pairs = (apss_pair*)std::malloc(sizeof(apss_pair) * std::max<size_t>(pairs.size(), 1));
-    if (!out->pairs) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
-    if (!pairs.empty()) std::memcpy(out->pairs, pairs.data(), sizeof(apss_pair) * pairs.size());
+    if (I.opts.sort_output) sort_pairs(pairs.p, pairs.n);
+    out->count = pairs.n;
+    out->pairs = pairs.release();
     out->on_device = 0;
     out->p_used = I.p;
     out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

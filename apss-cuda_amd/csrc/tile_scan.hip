@@ -261,6 +261,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ unsigned long long obuf_base;
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
   __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
+  __shared__ double tI[BAND_ROWS];                   // threshold of each band row
   __shared__ float vI[Z::CI];                        // band entry values * 2^S, [q][tid]
   __shared__ uint4 dA[DM ? BAND_ROWS * 16 : 1];      // head records of the band
   __shared__ uint4 dB[DM ? TILE_ROWS * 16 : 1];      // head records of tile J
@@ -335,6 +336,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     normI[tid] = ok ? a.I.norm[r] : 0.f;
     hnI[tid] = ok ? a.I.hnorm[r] : 0.f;
     lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
+    tI[tid] = (ok && a.row_t) ? a.row_t[r] : t_band;
     if constexpr (DM == DENSE_BF16) dnI[tid] = ok ? a.I.dnorm[r] : 0.f;
   }
   if constexpr (DM != DENSE_OFF) {  // the band's head records (zero beyond its rows)
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
                                        normJ[jl], a.qinv, 0.0);
         if constexpr (DM == DENSE_I8) eps0 += a.eps_dense;
         if constexpr (DM == DENSE_BF16) eps0 += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
-        if (bound + eps0 < t_band) continue;
+        if (bound + eps0 < tI[il]) continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
         const float h = head_dot(a.I.col + ai, a.I.val + ai, a.I.rp[ri + 1] - ai, a.J.col + aj,
@@ -719,10 +721,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         if constexpr (DM == DENSE_I8) eps += a.eps_dense;
         if constexpr (DM == DENSE_BF16) eps += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
         sd = (double)q * a.qinv;
-        if (sd - eps >= t_band) {
+        const double t_row = tI[il];
+        if (sd - eps >= t_row) {
           // certain hit; its score is final only if the bound meets the relative tolerance
           if (a.rescore_all || eps > a.score_rel * (sd - eps)) to_band = true; else emit = true;
-        } else if (sd + eps >= t_band) {
+        } else if (sd + eps >= t_row) {
           to_band = true;
         }
       }
@@ -798,6 +801,7 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
                                                  const int64_t* __restrict__ rpJ,
                                                  const int32_t* __restrict__ colJ,
                                                  const float* __restrict__ valJ, double t,
+                                                 const double* __restrict__ row_t,
                                                  int64_t i_off, int64_t j_off, apss_pair* out,
                                                  unsigned long long* out_cnt, int64_t out_cap) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -815,7 +819,7 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
       else if (cx > cy) ++y;
       else { s += (double)valI[x] * (double)valJ[y]; ++x; ++y; }
     }
-    emit = s >= t;
+    emit = s >= (row_t ? row_t[i] : t);
   }
   int tot;
   long long slot = wave_reserve(out_cnt, emit, &tot);

@@ -199,7 +199,7 @@ def leg_serving(args):
         for _ in range(args.serving_probe):
             q, t, size = make_batch()
             tk = srv.submit(*q, t)
-            srv.wait(tk)
+            srv.wait_info(tk, keep_pairs=False)
             rows_done += size
         closed_s = time.perf_counter() - t1
         cap = args.serving_probe / closed_s
@@ -228,7 +228,7 @@ def leg_serving(args):
                         tk = tickets[k]
                     k += 1
                     try:
-                        _, ms = srv.wait(tk)
+                        _, _, ms = srv.wait_info(tk, keep_pairs=False)
                         lats.append(ms)
                     except apss.ApssError as e:
                         errs.append(e.code)

@@ -7,7 +7,7 @@ import pytest
 
 import apss
 import oracle
-from helpers import check_pairs, subset
+from helpers import check_pairs, subset, tiny_matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,21 @@ def test_batched_with_split_and_small_bitmap_tier():
     res = apss.run_batched(inst, ts, p=50, dense_features=64)
     for (rp, col, val, d), t, (pairs, _) in zip(inst, ts, res):
         check_pairs(pairs, oracle.apss_fp64(rp, col, val, d, t))
+
+
+def test_batched_thresholds_are_per_instance():
+    # instance A holds a pair whose exact FP64 score s sits on A's threshold;
+    # instance B, in the same launch, has a much lower threshold.  Pairs near
+    # a threshold are settled by the FP64 rescore, which must use A's own.
+    rpA, colA, valA, dA = tiny_matrix([{2: 0.6, 5: 0.8}, {2: 0.8, 5: 0.6}, {5: 1.0}], 8)
+    s = oracle.pair_fp64(rpA, colA, valA, 0, 1)
+    B = small_rows(300, seed_offset=40)
+    A = (rpA, colA, valA, B[3])
+    for tA, expect in ((float(np.nextafter(s, 2.0)), False), (s, True)):
+        res = apss.run_batched([A, B], [tA, 0.2])
+        got = set(zip(res[0][0]["i"].tolist(), res[0][0]["j"].tolist()))
+        assert ((0, 1) in got) == expect
+        check_pairs(res[1][0], oracle.apss_fp64(*B, 0.2))
 
 
 def test_batched_long_lists_across_instances():
@@ -85,8 +100,8 @@ def test_index_inserts_and_merge():
 def test_server_result_same_alone_and_in_the_mix():
     crp, ccol, cval, d = small_rows(2500, name="serving")
     ix = apss.Index(crp, ccol, cval, d)
-    sizes = [1, 37, 600, 5, 250, 1200, 64, 3]
-    ts = [0.6, 0.8, 0.6, 0.5, 0.8, 0.6, 0.7, 0.9]
+    sizes = [1, 37, 600, 0, 5, 250, 1200, 64, 3]  # one request without rows
+    ts = [0.6, 0.8, 0.6, 0.7, 0.5, 0.8, 0.6, 0.7, 0.9]
     qs = [small_rows(n, seed_offset=400 + k, name="serving")[:3] for k, n in enumerate(sizes)]
     try:
         alone = [ix.query(*q, t)[0] for q, t in zip(qs, ts)]
