@@ -46,10 +46,10 @@ class Result(ctypes.Structure):
                 ("p_used", c_int32), ("cells", c_int64), ("band", c_int64),
                 ("products", c_int64), ("head_pairs", c_int64), ("list_visits", c_int64),
                 ("cursor_products", c_int64), ("mfma_features", c_int64),
-                ("dense_rows", c_int64), ("tile_pairs", c_int64),
+                ("dense_rows", c_int64), ("tile_pairs", c_int64), ("bitmap_features", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
-                ("internal_", c_void_p)]
+                ("ms_tune", c_double), ("internal_", c_void_p)]
 
 
 class GenSpec(ctypes.Structure):

@@ -1,5 +1,6 @@
 // Host-side engine of libapss.so: device memory, index build, scan launches.
 #pragma once
+#include <functional>
 #include <map>
 #include <memory>
 
@@ -36,7 +37,8 @@ __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2til
                              const int64_t* tstart, uint32_t* spos);
 template <int DM>
 __global__ void k_band_scan(ScanArgs a);
-__global__ void k_rescore(const unsigned long long* band, int64_t nband, const int64_t* rpI,
+__global__ void k_rescore(const unsigned long long* band, const unsigned long long* band_cnt,
+                          int64_t band_cap, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,
                           const int32_t* colJ, const float* valJ, double t, const double* row_t,
                           int64_t i_off, int64_t j_off, apss_pair* out,
@@ -176,7 +178,35 @@ struct RunConfig {
   const double* row_t = nullptr;       // device, per I row: threshold (NULL: t)
   double t_min = 0.0;                  // smallest per-band threshold
   int64_t out_cap_hint = 0;            // initial output capacity in pairs (0: from free memory)
+  bool use_graph = false;              // run the device job (counters, scan, rescore) as one
+                                       // captured hipGraph
+  bool tune_only = false;              // only time the sampled count pass (ms_scan = the
+                                       // estimated full scan time)
 };
+
+// Configuration of the tail tiers chosen by the autotuner (p < 0).
+struct TuneChoice {
+  int32_t dm = DENSE_OFF;  // dense MFMA head mode
+  int32_t kd_cap = 0;      // cap of the bitmap tier (0: the mode's maximum)
+  int32_t candidates = 0;  // configurations timed
+  double est_ms = 0.0;     // estimated scan time of the choice
+  double ms = 0.0;         // time spent tuning
+};
+
+// Everything a configuration builds on top of a prepared side.
+struct Built {
+  Tiers tr;
+  TileSet bands, tiles;
+};
+// Tiers, band and tile tilings, cursor tiles and dense records of one configuration.
+Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
+                   int32_t dm, const std::vector<int64_t>& bounds);
+// Autotuner: times the sampled count-only scan of each candidate configuration
+// (dense head on / off, bitmap tier at its maximum / 1024 features) and returns
+// the fastest.  `cfg` fills the run configuration of a candidate's tiling.
+TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int32_t self,
+                    const std::vector<int64_t>& bounds,
+                    const std::function<void(RunConfig&, const Built&)>& cfg);
 
 struct ScanOutput {
   apss_pair* pairs = nullptr;   // device, owned by the arena

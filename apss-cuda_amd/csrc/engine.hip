@@ -573,6 +573,52 @@ void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
   }
 }
 
+Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
+                   int32_t dm, const std::vector<int64_t>& bounds) {
+  Built b;
+  b.tr = build_tiers(ar, st, s, p, self, kd_cap, dm);
+  b.bands = build_tiles(ar, st, s, b.tr, self, BAND_ROWS, cap_i_for(b.tr.dm), CAP_S, true, bounds);
+  b.tiles = build_tiles(ar, st, s, b.tr, self, TILE_ROWS, cap_j_for(b.tr.dm), INT32_MAX, false, bounds);
+  set_cursor_tiles(ar, st, b.tr, b.tiles);
+  build_dense(ar, st, s, b.tr);
+  return b;
+}
+
+TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int32_t self,
+                    const std::vector<int64_t>& bounds,
+                    const std::function<void(RunConfig&, const Built&)>& cfg) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int32_t head = dense_mode(o);
+  std::vector<std::pair<int32_t, int32_t>> cand;  // {dense mode, bitmap cap}
+  if (head != DENSE_OFF) cand.push_back({head, 0});
+  cand.push_back({DENSE_OFF, 0});
+  if (head != DENSE_OFF) cand.push_back({head, 1024});
+  cand.push_back({DENSE_OFF, 1024});
+  TuneChoice best;
+  best.est_ms = INFINITY;
+  for (const auto& c : cand) {
+    Arena car(o.device, 0);  // the candidate's structures are freed after timing
+    Side sc = s;
+    Built b = build_config(car, st, sc, p, self, c.second, c.first, bounds);
+    RunConfig rc;
+    cfg(rc, b);
+    rc.tune_only = true;
+    ScanOutput so = scan_and_rescore(car, st, sc, b.bands, sc, b.tiles, b.tr, rc);
+    ++best.candidates;
+    if (trace_on())
+      std::fprintf(stderr, "[apss] tune: dense %d bitmap cap %d -> %.2f ms\n", c.first, c.second,
+                   so.ms_scan);
+    if (so.ms_scan < best.est_ms) {
+      best.est_ms = so.ms_scan;
+      best.dm = b.tr.dm;
+      best.kd_cap = c.second;
+    }
+  }
+  APSS_HIP(hipStreamSynchronize(st));
+  best.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return best;
+}
+
 // ---------------------------------------------------------------------------
 // scan + rescore
 
@@ -751,10 +797,13 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   if (rc.out_cap_hint > 0) {
     cap = (int64_t)std::min<double>(cells, (double)rc.out_cap_hint);
     bcap = std::max<int64_t>(1 << 20, cap / 64);
-  } else if (cells <= (double)(1 << 24) || bands.T < 64) {
+  } else if (!rc.tune_only && (cells <= (double)(1 << 24) || bands.T < 64)) {
     cap = (int64_t)std::min(cells, room);
     bcap = std::max<int64_t>(1 << 20, std::min<int64_t>(cap, 1 << 24));
   } else {
+    // count-only launch over a sample of the bands (every stride-th band
+    // against all its partner tiles, in 64-tile chunks); its time, scaled by
+    // tile pairs, is also the autotuner's cost estimate of the configuration
     const int32_t stride = std::max<int32_t>(1, bands.T / 128);
     std::vector<int4> sitems;
     const int64_t stp = cut_items(64, stride, sitems);
@@ -766,12 +815,21 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.out_cap = 0;
     a.band = nullptr;
     a.band_cap = 0;
+    Timer ts(st);
+    ts.start();
     launch(std::max(1, std::min(a.n_items, n_cu)));
+    const double sample_ms = ts.stop_ms();
     unsigned long long h2[2];
     APSS_HIP(hipMemcpyAsync(h2, ctr, sizeof(h2), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     ar.release(sdev);
     const double scale = stp > 0 ? (double)partner_tiles / (double)stp : 1.0;
+    if (rc.tune_only) {
+      so.ms_scan = sample_ms * scale;
+      so.tile_pairs = partner_tiles;
+      so.count = (int64_t)((double)h2[0] * scale);
+      return so;
+    }
     cap = (int64_t)std::min(cells, std::min(room, (double)h2[0] * scale * 1.25 + (double)(1 << 22)));
     bcap = (int64_t)((double)h2[1] * scale * 1.25) + (1 << 20);
     ph.mark("output estimate");
@@ -780,8 +838,28 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.items = items;
   a.n_items = n_items;
 
-  Timer tm(st);
   ph.mark("scan set-up");
+  // The device job: clear the counters, band scan, FP64 rescore of the band
+  // list (sized on the device).  Eagerly with events around the two kernels,
+  // or captured once as a hipGraph and launched as one unit.
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() { for (int k = 0; k < 3; ++k) if (e[k]) (void)hipEventDestroy(e[k]); }
+  } evg{ev};
+  for (int k = 0; k < 3; ++k) APSS_HIP(hipEventCreate(&ev[k]));
+  const int rs_grid = std::max(1, std::min(4 * n_cu * 4, 1 << 14));
+  auto enqueue = [&](apss_pair* out, int64_t ocap, unsigned long long* band, int64_t bc,
+                     bool timed) {
+    if (timed) APSS_HIP(hipEventRecord(ev[0], st));
+    launch(grid);
+    if (timed) APSS_HIP(hipEventRecord(ev[1], st));
+    hipLaunchKernelGGL(k_rescore, dim3(rs_grid), dim3(256), 0, st, band, ctr + 1, bc, I.rp, I.col,
+                       I.val, J.rp, J.col, J.val, rc.t, rc.row_t, rc.i_off, rc.j_off, out, ctr + 0,
+                       ocap);
+    APSS_HIP(hipGetLastError());
+    if (timed) APSS_HIP(hipEventRecord(ev[2], st));
+  };
   for (int attempt = 0; attempt < 3; ++attempt) {
     apss_pair* out = ar.alloc<apss_pair>(cap);
     unsigned long long* band = ar.alloc<unsigned long long>(bcap);
@@ -789,18 +867,55 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.out_cap = cap;
     a.band = band;
     a.band_cap = bcap;
-    tm.start();
-    launch(grid);
-    so.ms_scan = tm.stop_ms();
-    ph.mark("band scan");
+    if (rc.use_graph) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      APSS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(out, cap, band, bcap, false);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      APSS_HIP(hipStreamEndCapture(st, &g));
+      struct GraphGuard {
+        hipGraph_t* g;
+        hipGraphExec_t* ge;
+        ~GraphGuard() {
+          if (*ge) (void)hipGraphExecDestroy(*ge);
+          if (*g) (void)hipGraphDestroy(*g);
+        }
+      } gg{&g, &ge};
+      APSS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      APSS_HIP(hipEventRecord(ev[0], st));
+      APSS_HIP(hipGraphLaunch(ge, st));
+      APSS_HIP(hipEventRecord(ev[2], st));
+      APSS_HIP(hipEventSynchronize(ev[2]));
+      float ms = 0.f;
+      APSS_HIP(hipEventElapsedTime(&ms, ev[0], ev[2]));
+      so.ms_scan = ms;  // the whole graph (scan + rescore)
+      so.ms_rescore = 0.0;
+    } else {
+      enqueue(out, cap, band, bcap, true);
+      APSS_HIP(hipEventSynchronize(ev[2]));
+      float m1 = 0.f, m2 = 0.f;
+      APSS_HIP(hipEventElapsedTime(&m1, ev[0], ev[1]));
+      APSS_HIP(hipEventElapsedTime(&m2, ev[1], ev[2]));
+      so.ms_scan = m1;
+      so.ms_rescore = m2;
+    }
+    ph.mark("band scan + rescore");
     unsigned long long h[16];
     APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     const int64_t nout = (int64_t)h[0], nband = (int64_t)h[1];
-    if (nout > cap || nband > bcap) {  // capacity too small: grow and run again
+    if (nout > cap || nband > bcap) {
+      // capacity too small: every count is exact except the rescore's share of
+      // a truncated band list; grow and run again
       ar.release(out);
       ar.release(band);
-      cap = std::max(cap, nout + nband + 1024);
+      cap = std::max(cap, nout + std::max<int64_t>(0, nband - bcap) + 1024);
       bcap = std::max(bcap, nband + 1024);
       continue;
     }
@@ -813,25 +928,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.tile_pairs = partner_tiles;
     so.band = nband;
     so.n_dense = tr.n_dense;
-    if (nout + nband > cap) {  // room for every band pair before the rescore
-      apss_pair* bigger = ar.alloc<apss_pair>(nout + nband);
-      if (nout) APSS_HIP(hipMemcpyAsync(bigger, out, sizeof(apss_pair) * nout, hipMemcpyDeviceToDevice, st));
-      ar.release(out);
-      out = bigger;
-      cap = nout + nband;
-    }
-    tm.start();
-    if (nband > 0) {
-      hipLaunchKernelGGL(k_rescore, dim3(blocks_for(nband, 256, 1 << 30)), dim3(256), 0, st, band,
-                         nband, I.rp, I.col, I.val, J.rp, J.col, J.val, rc.t, rc.row_t, rc.i_off, rc.j_off,
-                         out, ctr + 0, cap);
-      APSS_HIP(hipGetLastError());
-    }
-    so.ms_rescore = tm.stop_ms();
-    ph.mark("rescore");
-    APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipStreamSynchronize(st));
-    so.count = (int64_t)h[0];
+    so.count = nout;
     so.pairs = out;
     ar.release(band);
     return so;
@@ -874,19 +971,35 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   } sg{st, own_stream};
   auto t0 = std::chrono::steady_clock::now();
   Arena ar(o->device, o->mem_budget);
-  if (p < 0) p = 0;  // autotune picks p = 0 for the FP32 path (all-tail scan)
+  // p < 0: autotune.  The split is p = 0 (every feature on the tiered
+  // inverted-list / dense-head path; p > 0 adds the reference's forward-list
+  // head dot, which loses on Zipf data); the tuner picks the tiering.
+  const bool tune = p < 0;
+  if (p < 0) p = 0;
   Timer tm(st);
   PhaseTrace ph;
   tm.start();
   Side s = prepare_side(ar, st, *X, p);
   out->ms_upload = tm.stop_ms();
   ph.mark("upload + validate");
+  int32_t dm = dense_mode(*o), kd_cap = o->dense_features;
+  if (tune) {
+    const TuneChoice tc = autotune(st, *o, s, p, 1, {}, [&](RunConfig& r, const Built&) {
+      r.t = t;
+      r.p = p;
+      r.self = 1;
+    });
+    dm = tc.dm;
+    kd_cap = tc.kd_cap;
+    out->ms_tune = tc.ms;
+    ph.mark("autotune");
+  }
   tm.start();
-  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features, dense_mode(*o));
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, cap_i_for(tr.dm), CAP_S, true, {});
-  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, cap_j_for(tr.dm), INT32_MAX, false, {});
-  set_cursor_tiles(ar, st, tr, tiles);
-  build_dense(ar, st, s, tr);
+  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, {});
+  Tiers& tr = bc.tr;
+  TileSet& bands = bc.bands;
+  TileSet& tiles = bc.tiles;
+  out->bitmap_features = tr.n_bitmap;
   out->ms_index = tm.stop_ms();
   ph.mark("index");
   RunConfig rc;
@@ -980,6 +1093,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   apss_opts_init(&def);
   if (!o) o = &def;
   if (count < 0 || (count > 0 && (!X || !t || !out))) throw Error{APSS_E_INVALID, "null argument"};
+  const bool tune = p < 0;  // autotune (see run_self)
   if (p < 0) p = 0;
   int64_t n_all = 0, nnz_all = 0, d = 0;
   double t_min = INFINITY;
@@ -1021,25 +1135,44 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   tm.start();
   Side s = prepare_side(ar, st, all, p);
   const double ms_upload = tm.stop_ms();
-  tm.start();
-  Tiers tr = build_tiers(ar, st, s, p, 1, o->dense_features, dense_mode(*o));
   std::vector<int64_t> bounds(off.begin() + 1, off.end() - 1);
   std::sort(bounds.begin(), bounds.end());
   bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
-  TileSet bands = build_tiles(ar, st, s, tr, 1, BAND_ROWS, cap_i_for(tr.dm), CAP_S, true, bounds);
-  TileSet tiles = build_tiles(ar, st, s, tr, 1, TILE_ROWS, cap_j_for(tr.dm), INT32_MAX, false, bounds);
-  set_cursor_tiles(ar, st, tr, tiles);
-  build_dense(ar, st, s, tr);
-  // per band: instance, partner range end, threshold
-  std::vector<int32_t> hjend((size_t)std::max(bands.T, 0));
-  std::vector<double> ht((size_t)std::max(bands.T, 1));
-  for (int32_t b = 0; b < bands.T; ++b) {
-    const int64_t r = bands.cuts[b];
-    const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), r) - off.begin()) - 1;
-    const int64_t end_row = off[k + 1];
-    hjend[b] = (int32_t)(std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) - tiles.cuts.begin());
-    ht[b] = t[k];
+  // per band: its instance's partner range end and threshold
+  auto band_ranges = [&](const TileSet& bands, const TileSet& tiles, std::vector<int32_t>& jend,
+                         std::vector<double>* bt) {
+    jend.assign((size_t)std::max(bands.T, 0), 0);
+    if (bt) bt->assign((size_t)std::max(bands.T, 1), t_min);
+    for (int32_t b = 0; b < bands.T; ++b) {
+      const int64_t r = bands.cuts[b];
+      const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), r) - off.begin()) - 1;
+      const int64_t end_row = off[k + 1];
+      jend[b] = (int32_t)(std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) -
+                          tiles.cuts.begin());
+      if (bt) (*bt)[b] = t[k];
+    }
+  };
+  int32_t dm = dense_mode(*o), kd_cap = o->dense_features;
+  double ms_tune = 0.0;
+  if (tune) {  // the tuner's candidates decide at the lowest threshold of the batch
+    const TuneChoice tc = autotune(st, *o, s, p, 1, bounds, [&](RunConfig& r, const Built& b) {
+      r.t = t_min;
+      r.p = p;
+      r.self = 1;
+      band_ranges(b.bands, b.tiles, r.band_jend, nullptr);
+    });
+    dm = tc.dm;
+    kd_cap = tc.kd_cap;
+    ms_tune = tc.ms;
   }
+  tm.start();
+  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, bounds);
+  Tiers& tr = bc.tr;
+  TileSet& bands = bc.bands;
+  TileSet& tiles = bc.tiles;
+  std::vector<int32_t> hjend;
+  std::vector<double> ht;
+  band_ranges(bands, tiles, hjend, &ht);
   double* dt = ar.alloc<double>(bands.T);
   if (bands.T > 0)
     APSS_HIP(hipMemcpyAsync(dt, ht.data(), sizeof(double) * bands.T, hipMemcpyHostToDevice, st));
@@ -1061,6 +1194,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   rc.band_t = dt;
   rc.row_t = drt;
   rc.t_min = t_min;
+  rc.use_graph = true;  // the batch's device job is one captured graph
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   // split the pair list by instance
   std::vector<apss_pair> h((size_t)so.count);
@@ -1103,6 +1237,9 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     out[k].ms_scan = so.ms_scan;
     out[k].ms_rescore = so.ms_rescore;
     out[k].ms_total = ms_total;
+    out[k].ms_tune = ms_tune;
+    out[k].bitmap_features = tr.n_bitmap;
+    out[k].mfma_features = tr.n_dense;
   }
   return APSS_OK;
 }

@@ -794,8 +794,12 @@ template __global__ void k_band_scan<DENSE_BF16>(ScanArgs);
 
 // FP64 rescore of the band: exact products summed in ascending feature id,
 // the same order as the FP64 oracle, so the decision s >= t is settled exactly.
+// Grid-stride over the band count the scan left in device memory (capped at
+// the list's capacity), so the scan and the rescore follow each other on the
+// stream with no host round trip and can be captured in one graph.
 __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __restrict__ band,
-                                                 int64_t nband, const int64_t* __restrict__ rpI,
+                                                 const unsigned long long* __restrict__ band_cnt,
+                                                 int64_t band_cap, const int64_t* __restrict__ rpI,
                                                  const int32_t* __restrict__ colI,
                                                  const float* __restrict__ valI,
                                                  const int64_t* __restrict__ rpJ,
@@ -804,7 +808,13 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
                                                  const double* __restrict__ row_t,
                                                  int64_t i_off, int64_t j_off, apss_pair* out,
                                                  unsigned long long* out_cnt, int64_t out_cap) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)*band_cnt;
+  const int64_t nband = c < band_cap ? c : band_cap;
+  // the loop bound is uniform per workgroup: every lane takes part in the
+  // wave-wide reservation below
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nband;
+       base += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t k = base + threadIdx.x;
   bool emit = false;
   uint32_t i = 0, j = 0;
   double s = 0.0;
@@ -829,6 +839,7 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
     p.j = (uint32_t)(j + j_off);
     p.score = (float)s;
     out[slot] = p;
+  }
   }
 }
 

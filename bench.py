@@ -20,8 +20,13 @@ Legs printed on the same JSON line (each can be switched off):
   mixed     the mixed-precision configuration (n = d = 5e5, mean 80 nnz/row,
             Zipf(1.2), uniform values, t = 0.5): plain path, exact int8 head,
             BF16 head -- the pair sets must be identical;
+  split     the split p swept over {0, 1024, 4096, 16384} at t = 0.5 and 0.9 on
+            the first 50,000 rows of the large workload (p > 0 runs the
+            reference's prefix-norm filter + forward-list head dot); the pair
+            counts must not depend on p;
   batched   256 independent instances (n log-uniform in [500, 5e4], d = 1e5)
-            in one band-scan launch, against the same instances run one by one;
+            in one captured band-scan graph with the tiering autotuned (p = -1),
+            against the same instances run one by one;
   serving   2e6-row resident corpus (d = 1e6, mean 60 nnz/row, Zipf(1.1),
             lognormal), query batches of 1..4096 rows (log-uniform), t drawn
             from {0.6, 0.8}, Poisson arrivals at fractions of the measured
@@ -52,7 +57,7 @@ import numpy as np  # noqa: E402
 import apss  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-ALL_LEGS = ("t09", "variants", "mixed", "batched", "serving", "cpu")
+ALL_LEGS = ("t09", "variants", "split", "mixed", "batched", "serving", "cpu")
 
 
 def log(msg):
@@ -139,6 +144,23 @@ def leg_mixed(hip, stream):
     return out
 
 
+def leg_split(n=50_000):
+    """Split sweep on a slice of the large workload: time and pair count per p."""
+    rp, col, val, d = apss.workload("large", n=n)
+    out = {"rows": n, "sweep": []}
+    for t in (0.5, 0.9):
+        counts = set()
+        for p in (0, 1024, 4096, 16384):
+            _, info = apss.run(rp, col, val, d, t, p, output_on_device=1)
+            counts.add(info["count"])
+            out["sweep"].append({"t": t, "p": p, "scan_ms": round(info["ms_scan"], 1),
+                                 "wall_ms": round(info["ms_total"], 1), "pairs": info["count"],
+                                 "head_pairs": info["head_pairs"]})
+            log(f"split t={t} p={p}: {info['ms_scan']:.1f} ms, {info['count']} pairs")
+        out[f"pairs_independent_of_p_t{t}"] = len(counts) == 1
+    return out
+
+
 def leg_batched():
     """256 instances in one launch sequence vs the same instances one by one."""
     t0 = time.time()
@@ -150,8 +172,9 @@ def leg_batched():
     ts = [0.8] * len(inst)
     apss.run_batched(inst[:4], ts[:4])  # warm-up
     t1 = time.perf_counter()
-    res = apss.run_batched(inst, ts)
+    res = apss.run_batched(inst, ts, p=-1)
     batched_s = time.perf_counter() - t1
+    tuned = res[0][1] if res else {}
     pairs_b = sum(info["count"] for _, info in res)
     t1 = time.perf_counter()
     pairs_s = 0
@@ -163,7 +186,10 @@ def leg_batched():
             "batched_wall_ms": round(batched_s * 1e3, 1), "one_by_one_wall_ms": round(single_s * 1e3, 1),
             "speedup": round(single_s / batched_s, 3), "pairs": pairs_b, "pairs_equal": pairs_b == pairs_s,
             "pair_cells_per_s": cells / batched_s,
-            "scan_ms": round(res[0][1]["ms_scan"], 1) if res else 0.0}
+            "graph_ms": round(tuned.get("ms_scan", 0.0), 1),
+            "autotune": {"ms": round(tuned.get("ms_tune", 0.0), 1), "p": tuned.get("p_used"),
+                         "mfma_features": tuned.get("mfma_features"),
+                         "bitmap_features": tuned.get("bitmap_features")}}
 
 
 def leg_serving(args):
@@ -293,8 +319,8 @@ def main():
     ap.add_argument("--serving-rows", type=int, default=2_000_000)
     ap.add_argument("--serving-budget", type=int, default=64 << 30)
     ap.add_argument("--serving-probe", type=int, default=24)
-    ap.add_argument("--serving-rates", type=float, nargs="*", default=[0.5, 0.9])
-    ap.add_argument("--serving-seconds", type=float, default=8.0)
+    ap.add_argument("--serving-rates", type=float, nargs="*", default=[0.25, 0.5, 0.75, 0.9])
+    ap.add_argument("--serving-seconds", type=float, default=6.0)
     ap.add_argument("--serving-min", type=int, default=40)
     ap.add_argument("--serving-max", type=int, default=400)
     args = ap.parse_args()
@@ -376,7 +402,7 @@ def main():
     X.release()
     del rp, col, val
 
-    for name, fn in (("mixed", lambda: leg_mixed(hip, stream)), ("batched", leg_batched),
+    for name, fn in (("split", leg_split), ("mixed", lambda: leg_mixed(hip, stream)), ("batched", leg_batched),
                      ("serving", lambda: leg_serving(args))):
         if name in skip:
             continue

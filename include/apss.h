@@ -111,8 +111,10 @@ typedef struct {
   int64_t mfma_features;   /* features in the dense MFMA head (0: none) */
   int64_t dense_rows;      /* 256-byte dense head records fetched by the scan */
   int64_t tile_pairs;      /* band x tile pairs the scan visited */
+  int64_t bitmap_features; /* features in the bitmap tier of the tail */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
+  double ms_tune;          /* autotuner (p < 0): time spent timing candidate configurations */
   void* internal_;
 } apss_result;
 
@@ -124,7 +126,10 @@ const char* apss_strerror(int code);
 const char* apss_last_error(void);
 void apss_opts_init(apss_opts* o);
 
-/* Self-join APSS of X at threshold t with head/tail split p (p = -1: autotune).
+/* Self-join APSS of X at threshold t with head/tail split p.  p < 0 autotunes:
+ * the split is 0 and the tail's tiering (dense MFMA head on / off, bitmap tier
+ * size) is the fastest of the candidates timed on a sampled count-only scan.
+ * The pair set does not depend on the choice.
  * Replaces similarity_join_device (similarity_search.h:3-14) for one matrix. */
 int apss_run(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
 

@@ -50,6 +50,32 @@ def test_batched_thresholds_are_per_instance():
         check_pairs(res[1][0], oracle.apss_fp64(*B, 0.2))
 
 
+def test_batched_graph_repeated_with_eager_work_between():
+    # every batched call runs its device job as one captured graph; repeat it
+    # with eager runs and synchronisations in between, checking every result
+    inst = [small_rows(n, seed_offset=50 + k) for k, n in enumerate([900, 400, 1300])]
+    ts = [0.5, 0.6, 0.45]
+    exp = [oracle.apss_fp64(*x, t) for x, t in zip(inst, ts)]
+    for rep in range(3):
+        res = apss.run_batched(inst, ts, p=0)
+        for (pairs, _), e in zip(res, exp):
+            check_pairs(pairs, e)
+        got, _ = apss.run(*inst[rep], ts[rep], 0)  # eager work between the graph launches
+        check_pairs(got, exp[rep])
+
+
+def test_autotuned_runs_match_oracle():
+    rp, col, val, d = small_rows(3000)
+    got, info = apss.run(rp, col, val, d, 0.5, -1)
+    check_pairs(got, oracle.apss_fp64(rp, col, val, d, 0.5))
+    assert info["p_used"] == 0 and info["ms_tune"] > 0
+    inst = [small_rows(n, seed_offset=60 + k, name="large") for k, n in enumerate([1200, 700])]
+    res = apss.run_batched(inst, [0.4, 0.5], p=-1)
+    for x, t, (pairs, info) in zip(inst, [0.4, 0.5], res):
+        check_pairs(pairs, oracle.apss_fp64(*x, t))
+        assert info["ms_tune"] > 0
+
+
 def test_batched_long_lists_across_instances():
     # instances of the batched workload (scaled down): concatenated, nine of
     # their common features form lists longer than one sort piece; a
