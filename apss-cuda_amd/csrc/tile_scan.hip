@@ -314,7 +314,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // Entry k = tid + q * SCAN_THREADS of the band block: (band row, bitmap
   // index, value * 2^S).  The block is ordered by frequency rank, so
   // the lanes of a wave hold entries of the same or similar features.
-  int32_t e_rf[Z::IPT];  // band row << 12 | bitmap index (or -1)
+  int32_t e_rf[Z::IPT];  // band row << 12 | bitmap index (KD past the block)
   {
     const int64_t d0 = a.I.dbase[I];
     const int nE = a.I.dseg[(int64_t)I * a.I.kds + a.KD];  // entries of the band block
@@ -326,7 +326,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       // the tile segment once in the list-visit statistics)
       const int32_t f = ok ? (int32_t)a.I.dfeat[d0 + k] : 0;
       const bool seg_head = ok && (k == 0 || (int32_t)a.I.dfeat[d0 + k - 1] != f);
-      e_rf[q] = ok ? ((((int32_t)a.I.drow[d0 + k]) << 12) | f | (seg_head ? (1 << 26) : 0)) : -1;
+      // entries past the block point at bitmap index KD, whose segment is
+      // empty in every tile (the segment-start rows end with the block total)
+      e_rf[q] = ok ? ((((int32_t)a.I.drow[d0 + k]) << 12) | f | (seg_head ? (1 << 26) : 0)) : a.KD;
       vI[k] = ok ? a.I.dval[d0 + k] * a.qscale : 0.f;  // exact: power of 2
     }
   }
@@ -481,10 +483,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int q = 0; q < Z::IPT / 2; ++q) {
         int rf = e_rf[h + q];
         asm volatile("" : "+v"(rf));  // no hoisting of per-entry values out of the J loop
-        const int f = rf >= 0 ? (rf & 0xfff) : 0;  // bitmap index (bits 0..11)
-        const int s0 = cJ[f], s1 = cJ[f + 1];
-        y0[q] = rf >= 0 ? s0 : 0;
-        y1[q] = rf >= 0 ? s1 : 0;
+        const int f = rf & 0xfff;  // bitmap index (bits 0..11)
+        y0[q] = cJ[f];
+        y1[q] = cJ[f + 1];
       }
 #pragma unroll
       for (int q = 0; q < Z::IPT / 2; ++q) {
@@ -665,22 +666,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         cmask |= ((m & 3) < lim[m >> 2] && dv[m] >= q_lo) ? (1u << m) : 0u;
     }
     unsigned ovf = 0;
-    if (__ballot(cmask != 0)) {  // rare: append this wave's candidates to the list
+    if (__ballot(cmask != 0)) {  // rare: append this lane's candidates to the list
+      unsigned int pos = 0;
+      if (cmask) pos = atomicAdd(&cand_n, (unsigned)__popc(cmask));  // one reservation per lane
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const bool c = (cmask >> m) & 1u;
-        const unsigned long long mc = __ballot(c);
-        if (!mc) continue;
-        const int leader = __ffsll((long long)mc) - 1;
-        unsigned int base = 0;
-        if (ln == leader) base = atomicAdd(&cand_n, (unsigned)__popcll(mc));
-        const unsigned int pos = __shfl(base, leader) + (unsigned)lane_rank(mc);
+        if (!((cmask >> m) & 1u)) continue;
         const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
-        if (c && pos < (unsigned)Z::CJ) cand[pos] = make_int2((il << 8) | jl, dv[m]);
-        if (c && pos >= (unsigned)Z::CJ) {  // list full: the cell keeps its score
+        if (pos < (unsigned)Z::CJ) {
+          cand[pos] = make_int2((il << 8) | jl, dv[m]);
+        } else {  // list full: the cell keeps its score
           tile[jl * TSTRIDE + il] = dv[m];
           ovf |= 1u << m;
         }
+        ++pos;
       }
     }
     lds_barrier();
