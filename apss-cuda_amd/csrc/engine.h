@@ -203,7 +203,8 @@ Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, 
                    int32_t dm, const std::vector<int64_t>& bounds);
 // Autotuner: times the sampled count-only scan of each candidate configuration
 // (dense head on / off, bitmap tier at its maximum / 1024 features) and returns
-// the fastest.  `cfg` fills the run configuration of a candidate's tiling.
+// the fastest (timed on a prefix of the rows).  `cfg` fills the run
+// configuration of a candidate's tiling.
 TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int32_t self,
                     const std::vector<int64_t>& bounds,
                     const std::function<void(RunConfig&, const Built&)>& cfg);

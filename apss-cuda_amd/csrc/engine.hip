@@ -594,12 +594,29 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int3
   cand.push_back({DENSE_OFF, 0});
   if (head != DENSE_OFF) cand.push_back({head, 1024});
   cand.push_back({DENSE_OFF, 1024});
+  // Candidates are timed on a prefix of the rows (1/16, at least 16384 rows):
+  // the tiers keep their relative shape (the bitmap-tier threshold scales with
+  // the tile count) and each candidate's build costs a fraction of the full one.
+  Side sub = s;
+  std::vector<int64_t> sbounds;
+  if (s.n > 32768) {
+    const int64_t m = std::max<int64_t>(16384, s.n / 16);
+    int64_t e = 0;
+    APSS_HIP(hipMemcpyAsync(&e, s.rp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    sub.n = m;
+    sub.nnz = e;
+    for (int64_t x : bounds)
+      if (x < m) sbounds.push_back(x);
+  } else {
+    sbounds = bounds;
+  }
   TuneChoice best;
   best.est_ms = INFINITY;
   for (const auto& c : cand) {
     Arena car(o.device, 0);  // the candidate's structures are freed after timing
-    Side sc = s;
-    Built b = build_config(car, st, sc, p, self, c.second, c.first, bounds);
+    Side sc = sub;
+    Built b = build_config(car, st, sc, p, self, c.second, c.first, sbounds);
     RunConfig rc;
     cfg(rc, b);
     rc.tune_only = true;
