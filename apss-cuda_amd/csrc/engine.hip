@@ -1127,29 +1127,38 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   }
   if (count == 0) return APSS_OK;
   auto t0 = std::chrono::steady_clock::now();
-  // concatenation on the host
+  // One device CSR holds every instance: the rebased row pointers are built on
+  // the host, the column ids and values go straight from each instance's
+  // arrays into their slice of the device arrays.
   std::vector<int64_t> off((size_t)count + 1, 0), rp((size_t)n_all + 1, 0);
-  std::vector<int32_t> col((size_t)nnz_all);
-  std::vector<float> val((size_t)nnz_all);
   int64_t r0 = 0, e0 = 0;
   for (int32_t k = 0; k < count; ++k) {
     off[k] = r0;
     const apss_csr& A = *X[k];
     for (int64_t i = 0; i < A.n_rows; ++i) rp[r0 + i + 1] = e0 + A.row_ptr[i + 1];
-    if (A.nnz) {
-      std::memcpy(col.data() + e0, A.col, sizeof(int32_t) * A.nnz);
-      std::memcpy(val.data() + e0, A.val, sizeof(float) * A.nnz);
-    }
     r0 += A.n_rows;
     e0 += A.nnz;
   }
   off[count] = r0;
-  apss_csr all{n_all, d, nnz_all, rp.data(), col.data(), val.data(), 0, 0};
   StreamScope ss(o->stream, o->device);
   hipStream_t st = ss.s;
   Arena ar(o->device, o->mem_budget);
   Timer tm(st);
   tm.start();
+  int64_t* drp = ar.alloc<int64_t>(n_all + 1);
+  int32_t* dcol = ar.alloc<int32_t>(nnz_all);
+  float* dval = ar.alloc<float>(nnz_all);
+  APSS_HIP(hipMemcpyAsync(drp, rp.data(), sizeof(int64_t) * (n_all + 1), hipMemcpyHostToDevice, st));
+  e0 = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    const apss_csr& A = *X[k];
+    if (A.nnz) {
+      APSS_HIP(hipMemcpyAsync(dcol + e0, A.col, sizeof(int32_t) * A.nnz, hipMemcpyHostToDevice, st));
+      APSS_HIP(hipMemcpyAsync(dval + e0, A.val, sizeof(float) * A.nnz, hipMemcpyHostToDevice, st));
+    }
+    e0 += A.nnz;
+  }
+  apss_csr all{n_all, d, nnz_all, drp, dcol, dval, 1, 0};
   Side s = prepare_side(ar, st, all, p);
   const double ms_upload = tm.stop_ms();
   std::vector<int64_t> bounds(off.begin() + 1, off.end() - 1);
