@@ -65,6 +65,8 @@ ABI_SYMBOLS = [
     "apss_index_insert", "apss_index_query", "apss_index_rows", "apss_index_destroy",
     "apss_server_create", "apss_server_submit", "apss_server_wait", "apss_server_cancel",
     "apss_server_destroy", "similarity_join_device", "apss_compat_last_count",
+    "apss_load_text", "apss_save_text", "apss_save_binary", "apss_load_binary", "apss_csr_free",
+    "apss_write_pairs", "apss_compare_pair_files",
 ]
 
 _lib = None
@@ -519,3 +521,92 @@ def similarity_join_device(parts):
     L.similarity_join_device.restype = None
     L.similarity_join_device(*args)
     return L.apss_compat_last_count()
+
+
+# ---------------------------------------------------------------------------
+# files (host-only entry points)
+
+def _bind_io(L):
+    if getattr(L, "_io_bound", False):
+        return L
+    L.apss_load_text.argtypes = [c_char_p, c_int64, POINTER(CSR)]
+    L.apss_load_text.restype = c_int32
+    L.apss_save_text.argtypes = [c_char_p, POINTER(CSR)]
+    L.apss_save_text.restype = c_int32
+    L.apss_save_binary.argtypes = [c_char_p, POINTER(CSR)]
+    L.apss_save_binary.restype = c_int32
+    L.apss_load_binary.argtypes = [c_char_p, POINTER(CSR)]
+    L.apss_load_binary.restype = c_int32
+    L.apss_csr_free.argtypes = [POINTER(CSR)]
+    L.apss_csr_free.restype = None
+    L.apss_write_pairs.argtypes = [c_char_p, c_void_p, c_int64]
+    L.apss_write_pairs.restype = c_int32
+    L.apss_compare_pair_files.argtypes = [c_char_p, c_char_p, POINTER(c_int64), POINTER(c_int64)]
+    L.apss_compare_pair_files.restype = c_int64
+    L._io_bound = True
+    return L
+
+
+def _take_csr(L, X):
+    """Copy a library-owned CSR into numpy arrays and release it."""
+    try:
+        n, nnz = X.n_rows, X.nnz
+        rp = np.ctypeslib.as_array((c_int64 * (n + 1)).from_address(X.row_ptr)).copy()
+        col = (np.ctypeslib.as_array((c_int32 * nnz).from_address(X.col)).copy() if nnz
+               else np.zeros(0, np.int32))
+        val = (np.ctypeslib.as_array((c_float * nnz).from_address(X.val)).copy() if nnz
+               else np.zeros(0, np.float32))
+        return rp, col, val, int(X.n_cols)
+    finally:
+        L.apss_csr_free(ctypes.byref(X))
+
+
+def _check(L, rc):
+    if rc != APSS_OK:
+        raise ApssError(rc, L.apss_last_error().decode())
+
+
+def load_text(path, n_cols=0):
+    """Rows of a file in the reference's text format -> (row_ptr, col, val, d)."""
+    L = _bind_io(lib())
+    X = CSR()
+    _check(L, L.apss_load_text(os.fsencode(path), int(n_cols), ctypes.byref(X)))
+    return _take_csr(L, X)
+
+
+def save_text(path, rp, col, val, d):
+    L = _bind_io(lib())
+    rp, col, val = (np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(col, np.int32),
+                    np.ascontiguousarray(val, np.float32))
+    _check(L, L.apss_save_text(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
+
+
+def save_binary(path, rp, col, val, d):
+    L = _bind_io(lib())
+    rp, col, val = (np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(col, np.int32),
+                    np.ascontiguousarray(val, np.float32))
+    _check(L, L.apss_save_binary(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
+
+
+def load_binary(path):
+    L = _bind_io(lib())
+    X = CSR()
+    _check(L, L.apss_load_binary(os.fsencode(path), ctypes.byref(X)))
+    return _take_csr(L, X)
+
+
+def write_pairs(path, pairs):
+    """Write a PAIR_DTYPE array as "i j score" lines."""
+    L = _bind_io(lib())
+    pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
+    _check(L, L.apss_write_pairs(os.fsencode(path), pairs.ctypes.data if len(pairs) else None, len(pairs)))
+
+
+def compare_pair_files(a, b):
+    """(only_in_a, only_in_b) pair counts of two pair files."""
+    L = _bind_io(lib())
+    oa, ob = c_int64(0), c_int64(0)
+    rc = L.apss_compare_pair_files(os.fsencode(a), os.fsencode(b), ctypes.byref(oa), ctypes.byref(ob))
+    if rc < 0:
+        raise ApssError(int(rc), L.apss_last_error().decode())
+    return oa.value, ob.value

@@ -107,6 +107,32 @@ def cpu_baseline(rp, col, val, d, t, budget_s):
     }
 
 
+def cpu_serial(rp, col, val, d, t, budget_s):
+    """Serial inverted-index search (oracle/, one thread, FP32; the reference's
+    InvertedIndex_search) on a stratified row sample: a second CPU data point."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    n = len(rp) - 1
+    # a stratified grid in a fixed random order: any prefix is a sample of the
+    # whole range; the search stops after budget_s seconds
+    grid = np.linspace(0, n - 2, num=min(n - 1, 65536), dtype=np.int64)
+    grid = grid[np.random.default_rng(7).permutation(len(grid))]
+    cnt, sec, m = oracle.inverted_search_fp32(rp, col, val, d, t, grid, budget_s)
+    rows = grid[:m]
+    pairs = float(np.sum((n - 1 - rows).astype(np.float64)))
+    return {
+        "value": pairs / sec,
+        "unit": "pairs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{m} query rows (a seeded random prefix of a stratified grid over the {n} rows), each "
+                   f"scored against every row through "
+                   f"the full inverted index ({pairs:.3e} pair cells of j > i, {cnt} pairs, {sec:.1f} s); "
+                   f"serial inverted-index search, FP32"),
+    }
+
+
 def info_of(r):
     return {f: getattr(r, f) for f, _ in apss.Result._fields_ if f not in ("pairs", "internal_")}
 
@@ -399,6 +425,8 @@ def main():
     if "cpu" not in skip:
         log("cpu baseline ...")
         cpu = cpu_baseline(rp, col, val, d, args.t, args.cpu_seconds)
+        log("cpu serial inverted-index search ...")
+        extra["cpu_serial"] = cpu_serial(rp, col, val, d, args.t, args.cpu_seconds / 3)
     X.release()
     del rp, col, val
 

@@ -192,6 +192,32 @@ void similarity_join_device(int* dim_A, float* feature_A, int* tile_start_A, int
                             int* apss_seg_start);
 int64_t apss_compat_last_count(void);
 
+/* ---- Files (host only; no device calls) ----------------------------------
+ * Text rows in the reference's input format: one row per line, a leading
+ * separator character, then "d v" pairs up to the line end.  Replaces the
+ * per-partition reading loops of main.cu:286-733 (which fill SELL-32 arrays;
+ * convert with any CSR->SELL step the caller already has, or call apss_run).
+ * Rows keep file order; ids are sorted within a row, a repeated id is
+ * APSS_E_INVALID.  n_cols <= 0 takes max id + 1.  *out owns malloc'd arrays:
+ * release with apss_csr_free. */
+int apss_load_text(const char* path, int64_t n_cols, apss_csr* out);
+/* Writes a host CSR in the same text format (' ' leading separator). */
+int apss_save_text(const char* path, const apss_csr* X);
+/* Binary CSR: "APSSCSR1", int64 n_rows, n_cols, nnz, row_ptr[n_rows+1]
+ * (int64), col[nnz] (int32), val[nnz] (float32), little-endian.  No
+ * counterpart in the reference (its only input is the text file). */
+int apss_save_binary(const char* path, const apss_csr* X);
+int apss_load_binary(const char* path, apss_csr* out);
+void apss_csr_free(apss_csr* X);
+/* Pair file: one "i j score" line per pair, in the given order. */
+int apss_write_pairs(const char* path, const apss_pair* pairs, int64_t count);
+/* Compares the pair sets of two pair files (first two columns; (i, j) and
+ * (j, i) are the same pair).  Returns the number of pairs found in only one
+ * of them (0: same set) or a negative error code; only_a / only_b (optional)
+ * receive the two one-sided counts.  Replaces compare() (main.cu:143-171),
+ * which compares the first 22000 lines of two files line by line. */
+int64_t apss_compare_pair_files(const char* a, const char* b, int64_t* only_a, int64_t* only_b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,19 @@
  *                     stride 1024 vs 512 threads, half-covered Y range, prune
  *                     on `<= t`).  Pinned against oracle_apss_fp64: same pair
  *                     set except pairs within FP32 rounding of t.
+ *  oracle_inverted_search_fp32
+ *                     serial inverted-index search (main.cu:104-141): for
+ *                     each query row, every entry of the inverted list of
+ *                     each of its features (all rows, no split, no filter)
+ *                     adds vec_fea * x to an FP32 accumulator, the product
+ *                     and sum formed in double and rounded once (Term::val
+ *                     is float, the row's value and the sum are double,
+ *                     the accumulator std::map<int,float>); pairs >= t are
+ *                     counted (rows in the given order until budget_s
+ *                     seconds have passed, if budget_s > 0).  A dense accumulator with a touched list
+ *                     replaces std::map (same sums, same order).  Pinned
+ *                     against oracle_apss_fp64: same count except pairs
+ *                     within FP32 rounding of t.
  */
 #include <math.h>
 #include <stdint.h>
@@ -311,6 +324,55 @@ int oracle_hybrid_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* c
   if (!bufs) return -1;
   int rc = gather(bufs, nb, out);
   return err ? -1 : rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* serial inverted-index search (FP32), the second CPU data point            */
+
+int oracle_inverted_search_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* col,
+                                const float* val, double t, const int64_t* rows, int64_t nrows,
+                                double budget_s, int64_t* done, int64_t* count, double* seconds) {
+  int64_t nnz = rp[n];
+  int64_t* cptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(d + 1));
+  int32_t* crow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  float* cval = (float*)malloc(sizeof(float) * (size_t)(nnz ? nnz : 1));
+  float* acc = (float*)calloc((size_t)(n ? n : 1), sizeof(float));
+  uint8_t* seen = (uint8_t*)calloc((size_t)(n ? n : 1), 1);
+  uint32_t* touched = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+  int rc = (cptr && crow && cval && acc && seen && touched) ? 0 : -1;
+  int64_t cnt = 0;
+  double t0 = 0.0, t1 = 0.0;
+  if (rc == 0) {
+    oracle_transpose(n, d, rp, col, val, 0, cptr, crow, cval);
+    const float tf = (float)t;
+    t0 = omp_get_wtime();
+    int64_t nq = rows ? nrows : n, q = 0;
+    for (; q < nq; ++q) {
+      if (budget_s > 0.0 && omp_get_wtime() - t0 > budget_s) break;
+      int64_t i = rows ? rows[q] : q;
+      int64_t nt = 0;
+      for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+        const double x = (double)val[e];
+        for (int64_t k = cptr[col[e]]; k < cptr[col[e] + 1]; ++k) {
+          uint32_t j = (uint32_t)crow[k];
+          if (!seen[j]) { seen[j] = 1; touched[nt++] = j; }
+          acc[j] = (float)((double)acc[j] + (double)cval[k] * x);
+        }
+      }
+      for (int64_t k = 0; k < nt; ++k) {
+        uint32_t j = touched[k];
+        cnt += (j > (uint64_t)i && acc[j] >= tf);
+        acc[j] = 0.0f;
+        seen[j] = 0;
+      }
+    }
+    t1 = omp_get_wtime();
+    if (done) *done = q;
+  }
+  free(cptr); free(crow); free(cval); free(acc); free(seen); free(touched);
+  if (count) *count = cnt;
+  if (seconds) *seconds = t1 - t0;
+  return rc;
 }
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }

@@ -34,6 +34,13 @@ double oracle_pair_fp64(const int32_t* ac, const float* av, int64_t na, const in
 int oracle_hybrid_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* col,
                        const float* val, double t, int32_t p, const int64_t* rows, int64_t nrows,
                        oracle_pairs* out, oracle_hybrid_stats* st);
+/* Serial (one thread) inverted-index search: counts pairs (i, j > i) with
+ * FP32 score >= t for the query rows (NULL = all), taken in order until
+ * budget_s seconds have passed (budget_s <= 0: all); *done = rows searched;
+ * seconds excludes the index build. */
+int oracle_inverted_search_fp32(int64_t n, int64_t d, const int64_t* rp, const int32_t* col,
+                                const float* val, double t, const int64_t* rows, int64_t nrows,
+                                double budget_s, int64_t* done, int64_t* count, double* seconds);
 void oracle_pairs_free(oracle_pairs* p);
 int oracle_num_threads(void);
 

@@ -42,6 +42,9 @@ def lib():
                                        c_void_p, c_void_p]
         L.oracle_pair_fp64.restype = c_double
         L.oracle_pair_fp64.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64]
+        L.oracle_inverted_search_fp32.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_double,
+                                                  c_void_p, c_int64, c_double, POINTER(c_int64),
+                                                  POINTER(c_int64), POINTER(c_double)]
         L.oracle_pairs_free.argtypes = [POINTER(_Pairs)]
         L.oracle_num_threads.restype = c_int32
         _lib = L
@@ -100,6 +103,23 @@ def hybrid_fp32(rp, col, val, d, t, p, rows=None):
         raise MemoryError("oracle_hybrid_fp32 failed")
     return _take(out), dict(candidates=st.candidates, survivors=st.survivors, head_gathers=st.head_gathers,
                             search_seconds=st.search_seconds)
+
+
+def inverted_search_fp32(rp, col, val, d, t, rows=None, budget_s=0.0):
+    """Serial inverted-index search; returns (pair count, search seconds, rows
+    searched).  With budget_s > 0 the rows are taken in order until that many
+    seconds have passed."""
+    rp, prp = _c(np.asarray(rp, np.int64)); col, pc = _c(np.asarray(col, np.int32)); val, pv = _c(np.asarray(val, np.float32))
+    if rows is not None:
+        rows, prows = _c(np.asarray(rows, np.int64))
+        nr = len(rows)
+    else:
+        prows, nr = None, 0
+    cnt, sec, done = c_int64(0), c_double(0.0), c_int64(0)
+    if lib().oracle_inverted_search_fp32(len(rp) - 1, d, prp, pc, pv, float(t), prows, nr, float(budget_s),
+                                         ctypes.byref(done), ctypes.byref(cnt), ctypes.byref(sec)) != 0:
+        raise MemoryError("oracle_inverted_search_fp32 failed")
+    return cnt.value, sec.value, done.value
 
 
 def transpose(rp, col, val, d, p=0):

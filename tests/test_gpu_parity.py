@@ -216,3 +216,22 @@ def test_bf16_head_with_split_and_rescore_all():
     check_pairs_abs(got, exp, 7.9e-3)
     got, _ = apss.run(rp, col, val, d, 0.45, 0, precision=apss.APSS_PREC_BF16_HEAD, rescore_all=1)
     check_pairs(got, exp)  # every score from the FP64 rescore
+
+
+def test_text_file_to_pair_file(tmp_path):
+    # Rows written in the reference's text format, loaded through
+    # apss_load_text, joined on the GPU, written with apss_write_pairs and
+    # compared as a set with the oracle's pair file.
+    rp, col, val, d = apss.workload("small", n=3000)
+    src = str(tmp_path / "rows.txt")
+    apss.save_text(src, rp, col, val, d)
+    r2, c2, v2, d2 = apss.load_text(src, n_cols=d)
+    got, _ = apss.run(r2, c2, v2, d2, 0.5, 0)
+    exp = oracle.apss_fp64(rp, col, val, d, 0.5)
+    a, b = str(tmp_path / "gpu.txt"), str(tmp_path / "oracle.txt")
+    apss.write_pairs(a, got)
+    e = np.zeros(len(exp), apss.PAIR_DTYPE)
+    e["i"], e["j"], e["score"] = exp["i"], exp["j"], exp["score"]
+    apss.write_pairs(b, e)
+    assert len(got) > 0
+    assert apss.compare_pair_files(a, b) == (0, 0)

@@ -147,3 +147,29 @@ def test_hybrid_row_subset_is_a_restriction():
     part, _ = oracle.hybrid_fp32(rp, col, val, d, 0.5, 4096, rows)
     fs = {(int(i), int(j)) for i, j in zip(full["i"], full["j"]) if i in set(rows.tolist())}
     assert {(int(i), int(j)) for i, j in zip(part["i"], part["j"])} == fs
+
+
+@pytest.mark.parametrize("t", [0.3, 0.5, 0.8])
+def test_inverted_search_counts_match_gram(t):
+    z = np.load(os.path.join(GOLDEN, "pairs_small400_t0.3.npz"))
+    rp, col, val, d = z["rp"], z["col"], z["val"], int(z["d"])
+    G = dense_gram(rp, col, val, d)
+    iu = np.triu_indices(len(rp) - 1, 1)
+    s = G[iu]
+    exact = int(np.count_nonzero(s >= t))
+    near = int(np.count_nonzero(np.abs(s - t) < 1e-5))
+    cnt, sec, done = oracle.inverted_search_fp32(rp, col, val, d, t)
+    assert abs(cnt - exact) <= near and sec >= 0.0 and done == len(rp) - 1
+    # a row subset counts exactly its rows' pairs (j > i)
+    rows = np.array([0, 7, 123, 398], np.int64)
+    sub, _, _ = oracle.inverted_search_fp32(rp, col, val, d, t, rows)
+    mask = np.isin(iu[0], rows)
+    assert abs(sub - int(np.count_nonzero(s[mask] >= t))) <= int(np.count_nonzero(np.abs(s[mask] - t) < 1e-5))
+
+
+def test_inverted_search_closed_forms():
+    # identical rows score 1; disjoint rows 0; one shared feature a*b
+    rp, col, val, d = tiny_matrix([{1: 0.6, 3: 0.8}, {1: 0.6, 3: 0.8}, {5: 1.0}, {3: 1.0}], 6)
+    assert oracle.inverted_search_fp32(rp, col, val, d, 0.99)[0] == 1
+    assert oracle.inverted_search_fp32(rp, col, val, d, 0.8)[0] == 3      # (0,1), (0,3), (1,3)
+    assert oracle.inverted_search_fp32(rp, col, val, d, 0.81)[0] == 1
