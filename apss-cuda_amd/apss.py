@@ -561,16 +561,11 @@ def _take_csr(L, X):
         L.apss_csr_free(ctypes.byref(X))
 
 
-def _check(L, rc):
-    if rc != APSS_OK:
-        raise ApssError(rc, L.apss_last_error().decode())
-
-
 def load_text(path, n_cols=0):
     """Rows of a file in the reference's text format -> (row_ptr, col, val, d)."""
     L = _bind_io(lib())
     X = CSR()
-    _check(L, L.apss_load_text(os.fsencode(path), int(n_cols), ctypes.byref(X)))
+    _check(L.apss_load_text(os.fsencode(path), int(n_cols), ctypes.byref(X)))
     return _take_csr(L, X)
 
 
@@ -578,20 +573,20 @@ def save_text(path, rp, col, val, d):
     L = _bind_io(lib())
     rp, col, val = (np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(col, np.int32),
                     np.ascontiguousarray(val, np.float32))
-    _check(L, L.apss_save_text(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
+    _check(L.apss_save_text(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
 
 
 def save_binary(path, rp, col, val, d):
     L = _bind_io(lib())
     rp, col, val = (np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(col, np.int32),
                     np.ascontiguousarray(val, np.float32))
-    _check(L, L.apss_save_binary(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
+    _check(L.apss_save_binary(os.fsencode(path), ctypes.byref(make_csr(rp, col, val, d))))
 
 
 def load_binary(path):
     L = _bind_io(lib())
     X = CSR()
-    _check(L, L.apss_load_binary(os.fsencode(path), ctypes.byref(X)))
+    _check(L.apss_load_binary(os.fsencode(path), ctypes.byref(X)))
     return _take_csr(L, X)
 
 
@@ -599,7 +594,7 @@ def write_pairs(path, pairs):
     """Write a PAIR_DTYPE array as "i j score" lines."""
     L = _bind_io(lib())
     pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
-    _check(L, L.apss_write_pairs(os.fsencode(path), pairs.ctypes.data if len(pairs) else None, len(pairs)))
+    _check(L.apss_write_pairs(os.fsencode(path), pairs.ctypes.data if len(pairs) else None, len(pairs)))
 
 
 def compare_pair_files(a, b):
