@@ -245,6 +245,22 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
   }
 }
 
+// Copies the LDS output buffer to HBM at one reserved range and empties it.
+// Called by the whole workgroup with obuf_n stable (after a barrier); obuf_n
+// may exceed the capacity (those pairs went to HBM directly).
+__device__ __forceinline__ void flush_obuf(const ScanArgs& a, int tl, const apss_pair* obuf,
+                                           unsigned int* obuf_n, unsigned long long* obuf_base) {
+  const unsigned int nbuf = *obuf_n;
+  const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
+  if (tl == 0) *obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
+  lds_barrier();
+  const unsigned long long ob = *obuf_base;
+  for (unsigned int k = tl; k < nf; k += SCAN_THREADS)
+    if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
+  lds_barrier();
+  if (tl == 0) *obuf_n = 0;
+}
+
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
 // DM selects the dense head (DENSE_OFF, DENSE_I8, DENSE_BF16).
 template <int DM>
@@ -259,16 +275,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ apss_pair obuf[OBUF_CAP];               // emitted pairs waiting for a flush
   __shared__ unsigned int obuf_n;
   __shared__ unsigned long long obuf_base;
-  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[TILE_ROWS], hnJ[TILE_ROWS];
-  __shared__ int32_t lenI[BAND_ROWS], lenJ[TILE_ROWS];
+  // per-row values of tile J in two buffers (by J % 2): a wave that settles the
+  // cells of tile J may still read them while the next tile is written
+  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[2][TILE_ROWS], hnJ[2][TILE_ROWS];
+  __shared__ int32_t lenI[BAND_ROWS], lenJ[2][TILE_ROWS];
   __shared__ double tI[BAND_ROWS];                   // threshold of each band row
   __shared__ float vI[Z::CI];                        // band entry values * 2^S, [q][tid]
   __shared__ uint4 dA[DM ? BAND_ROWS * 16 : 1];      // head records of the band
   __shared__ uint4 dB[DM ? TILE_ROWS * 16 : 1];      // head records of tile J
-  __shared__ float dnI[DM == DENSE_BF16 ? BAND_ROWS : 1], dnJ[DM == DENSE_BF16 ? TILE_ROWS : 1];
+  __shared__ float dnI[DM == DENSE_BF16 ? BAND_ROWS : 1], dnJ[2][DM == DENSE_BF16 ? TILE_ROWS : 1];
   __shared__ int4 item_s;
   __shared__ int32_t ch_end_s;
-  __shared__ unsigned int cand_n;                    // candidate cells of the tile pair
   __shared__ unsigned int rec_next;                  // cursor records handed out so far
   __shared__ unsigned long long acc_products, acc_head, acc_visits, acc_cursor;
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
@@ -282,7 +299,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // the tile is cleared cell by cell by the threshold pass, so once is enough
   for (int c = tid0; c < TILE_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
-    acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0; cand_n = 0;
+    acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
   }
   unsigned int my_products = 0, my_visits = 0;
 
@@ -439,19 +456,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           eJ[4 * k + q] = make_int2((int)((pf.rows[i] >> (8 * q)) & 0xffu) * TSTRIDE, __float_as_int(vv[q]));
       }
     }
+    const int jb = J & 1;  // buffer of this tile's per-row values
     if (tl < 4 * TILE_ROWS) {  // rows past the tile read as zero
       const int row = tl & 63, which = tl >> 6;
       const bool ok = row < nJ;
       const float v = ok ? __uint_as_float(pf.aux) : 0.f;
-      if (which == 0) normJ[row] = v;
-      else if (which == 1) hnJ[row] = v;
-      else if (which == 2) lenJ[row] = ok ? (int32_t)(pf.rp1 - pf.rp0) : 0;
-      else if (DM == DENSE_BF16) dnJ[row] = v;
+      if (which == 0) normJ[jb][row] = v;
+      else if (which == 1) hnJ[jb][row] = v;
+      else if (which == 2) lenJ[jb][row] = ok ? (int32_t)(pf.rp1 - pf.rp0) : 0;
+      else if (DM == DENSE_BF16) dnJ[jb][row] = v;
     }
     if constexpr (DM != DENSE_OFF)
       dB[dslot(tl >> 4, tl & 15)] = (tl >> 4) < nJ ? pf.dense : make_uint4(0, 0, 0, 0);
 
     lds_barrier();
+
+    // flush the output buffer once half full: every append of the earlier tile
+    // pairs is complete here (the next appends come after the MFMA barrier,
+    // which also orders the reset of obuf_n before them)
+    if (obuf_n >= OBUF_CAP / 2) flush_obuf(a, tl, obuf, &obuf_n, &obuf_base);
 
     // this thread's first cursor product of tile J: its load completes behind
     // the bitmap-tier products.  It is issued before the prefetch of the next
@@ -604,11 +627,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         int32_t* cell = &tile[jl * TSTRIDE + il];
         const int32_t part = *cell;
         // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
-        const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jl];
-        double eps0 = cell_error_bound(lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl], normI[il],
-                                       normJ[jl], a.qinv, 0.0);
+        const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jb][jl];
+        double eps0 = cell_error_bound(lenI[il] < lenJ[jb][jl] ? lenI[il] : lenJ[jb][jl], normI[il],
+                                       normJ[jb][jl], a.qinv, 0.0);
         if constexpr (DM == DENSE_I8) eps0 += a.eps_dense;
-        if constexpr (DM == DENSE_BF16) eps0 += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
+        if constexpr (DM == DENSE_BF16) eps0 += a.c_bf * (double)dnI[il] * (double)dnJ[jb][jl] + a.qinv;
         if (bound + eps0 < tI[il]) continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
@@ -621,14 +644,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       lds_barrier();
     }
 
-    // ---- 4. threshold, compaction, reset -------------------------------------
-    // Pass 1 reads and clears this thread's eight cells (+ their dense parts)
-    // and appends the ones that can reach t to a candidate list in LDS (the
-    // tile's entry block, dead by now).  Pass 2 settles the candidates with one
-    // thread each.  Valid cells: row below the band height, column below the
-    // tile height, and for self-joins i0 + il < j0 + jl.  Candidates beyond the
-    // list capacity stay in their cells and are settled by their own thread.
-    int2* const cand = eJ;
+    // ---- 4. threshold and emission ---------------------------------------------
+    // Each thread reads and clears its eight cells (+ their dense parts) and
+    // settles the ones that can reach t itself, one candidate per lane per
+    // round.  Valid cells: row below the band height, column below the tile
+    // height, and for self-joins i0 + il < j0 + jl.  Emitted pairs go to the
+    // LDS output buffer, cells within the error bound to the band list.  No
+    // barrier follows: the next tile pair's first barrier orders the cleared
+    // cells and the buffer appends before their next use.
     const int64_t dj = j0 - i0;
     // Cells past the band or tile height receive no products and a zero head,
     // so with q_lo > 0 only the self-join diagonal needs an explicit validity
@@ -665,60 +688,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int m = 0; m < 8; ++m)
         cmask |= ((m & 3) < lim[m >> 2] && dv[m] >= q_lo) ? (1u << m) : 0u;
     }
-    unsigned ovf = 0;
-    if (__ballot(cmask != 0)) {  // rare: append this lane's candidates to the list
-      unsigned int pos = 0;
-      if (cmask) pos = atomicAdd(&cand_n, (unsigned)__popc(cmask));  // one reservation per lane
+    while (__ballot(cmask != 0)) {  // rare: rounds over the candidate cells
+      const bool is_cand = cmask != 0;
+      const int m = is_cand ? __builtin_ctz(cmask) : 0;
+      cmask &= cmask - 1u;
+      int32_t q = dv[0];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        if (!((cmask >> m) & 1u)) continue;
-        const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
-        if (pos < (unsigned)Z::CJ) {
-          cand[pos] = make_int2((il << 8) | jl, dv[m]);
-        } else {  // list full: the cell keeps its score
-          tile[jl * TSTRIDE + il] = dv[m];
-          ovf |= 1u << m;
-        }
-        ++pos;
-      }
-    }
-    lds_barrier();
-    const int n_list = (int)(cand_n < (unsigned)Z::CJ ? cand_n : (unsigned)Z::CJ);
-    const int n_ovf = (int)cand_n - n_list;
-    for (int k0c = 0; k0c < n_list + (n_ovf > 0 ? 8 * SCAN_THREADS : 0); k0c += SCAN_THREADS) {
-      // rounds over the list, then (only after an overflow) one round per own cell
-      const int kk = k0c + tl;
-      bool is_cand = false;
-      int il = 0, jl = 0;
-      int32_t q = 0;
-      if (k0c < n_list) {
-        if (kk < n_list) {
-          const int2 r = cand[kk];
-          il = r.x >> 8;
-          jl = r.x & 0xff;
-          q = r.y;
-          is_cand = true;
-        }
-      } else {
-        const int m = (k0c - n_list) / SCAN_THREADS;
-        il = il_b + (m & 3);
-        jl = jl_b + 16 * (m >> 2);
-        if ((ovf >> m) & 1u) {
-          int32_t* cell = &tile[jl * TSTRIDE + il];
-          q = *cell;
-          *cell = 0;
-          is_cand = true;
-        }
-      }
-      if (!__ballot(is_cand)) continue;
+      for (int k = 1; k < 8; ++k) q = m == k ? dv[k] : q;
+      const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
       bool emit = false, to_band = false;
       double sd = 0.0;
       if (is_cand) {
-        const int k = lenI[il] < lenJ[jl] ? lenI[il] : lenJ[jl];
-        const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jl] : 0.0;
-        double eps = cell_error_bound(k, normI[il], normJ[jl], a.qinv, hb);
+        const int k = lenI[il] < lenJ[jb][jl] ? lenI[il] : lenJ[jb][jl];
+        const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jb][jl] : 0.0;
+        double eps = cell_error_bound(k, normI[il], normJ[jb][jl], a.qinv, hb);
         if constexpr (DM == DENSE_I8) eps += a.eps_dense;
-        if constexpr (DM == DENSE_BF16) eps += a.c_bf * (double)dnI[il] * (double)dnJ[jl] + a.qinv;
+        if constexpr (DM == DENSE_BF16) eps += a.c_bf * (double)dnI[il] * (double)dnJ[jb][jl] + a.qinv;
         sd = (double)q * a.qinv;
         const double t_row = tI[il];
         if (sd - eps >= t_row) {
@@ -757,25 +742,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       if (to_band && bslot < a.band_cap)
         a.band[bslot] = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
     }
-    lds_barrier();
-    if (tl == 0) cand_n = 0;  // read by every thread before the barrier above
-    // flush the LDS output buffer when half full, and after the last tile
-    const unsigned int nbuf = obuf_n;
-    if (nbuf >= OBUF_CAP / 2 || (J + 1 == J_end && nbuf > 0)) {
-      const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
-      if (tl == 0) obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
-      lds_barrier();
-      const unsigned long long ob = obuf_base;
-      for (unsigned int k = tl; k < nf; k += SCAN_THREADS)
-        if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
-      lds_barrier();
-      if (tl == 0) obuf_n = 0;
-      lds_barrier();
-    }
   }
   __syncthreads();
   }  // band queue
 
+  // pairs still in the output buffer (the barrier above ends every append)
+  if (obuf_n > 0) flush_obuf(a, tid0, obuf, &obuf_n, &obuf_base);
   atomicAdd(&acc_products, (unsigned long long)my_products);
   atomicAdd(&acc_visits, (unsigned long long)my_visits);
   __syncthreads();
