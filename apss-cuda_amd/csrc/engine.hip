@@ -333,7 +333,8 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
     throw Error{APSS_E_UNSUPPORTED, "cursor tier larger than 2^31 entries"};
   tr.fmap = ar.alloc<int32_t>(d);
   tr.sptr = ar.alloc<int64_t>(d + 1);
-  tr.sent = ar.alloc<int2>(tr.n_cursor_entries);
+  // padded: the band kernel's list walks read aligned groups of eight entries
+  tr.sent = ar.alloc<int2>(tr.n_cursor_entries + WALK_PAD);
   APSS_HIP(hipMemcpyAsync(tr.fmap, fm.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
   APSS_HIP(hipMemcpyAsync(tr.sptr, sp.data(), sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
   if (tr.n_cursor_entries > 0) {
@@ -548,7 +549,7 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
 
 void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles) {
   if (tr.spos) ar.release(tr.spos);
-  tr.spos = ar.alloc<uint32_t>(tr.n_cursor_entries);
+  tr.spos = ar.alloc<uint32_t>(tr.n_cursor_entries + WALK_PAD);
   if (tr.n_cursor_entries > 0) {
     hipLaunchKernelGGL(k_cursor_pos, dim3(blocks_for(tr.n_cursor_entries, 256, 65536)), dim3(256),
                        0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tiles.tstart, tr.spos);

@@ -178,28 +178,29 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, int tl, uint32_t* w
 
 // Cursor walks over one record's list from position cur, visiting the entries
 // whose J tile lies below ce (lists are row-sorted, so those come first).
-// Positions are read eight at a time so one memory round trip covers up to
-// eight entries: the eight loads are unconditional, from positions clamped to
-// the list's last entry (callers guarantee cur < end), and entries past the
-// end are masked afterwards, so the loads issue together and are waited for
-// once.
+// Entries are read as aligned 16-byte words, eight per step from the multiple
+// of four at or below cur: each lane's step touches one or two cache lines
+// instead of eight separate words.  Entries before cur are skipped (first step
+// only), entries at or after end stop the walk; the list arrays are padded by
+// eight entries, so a step never reads past them.  Callers guarantee cur < end.
 constexpr int WALK_BATCH = 8;
+static_assert(WALK_PAD >= WALK_BATCH, "a walk step reads up to seven entries past end - 1");
 
 __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
                                            int32_t cs, int32_t ce, uint32_t* bkt) {
-  for (;;) {
-    uint32_t pt[WALK_BATCH];
-#pragma unroll
-    for (int q = 0; q < WALK_BATCH; ++q) pt[q] = pos[cur + q < end ? cur + q : end - 1];
+  const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
+  for (int b = cur & ~3;; b += WALK_BATCH) {
+    const uint4 w0 = pos4[b >> 2], w1 = pos4[(b >> 2) + 1];
+    const uint32_t pt[WALK_BATCH] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
     bool go = true;
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
-      const int32_t t = cur + q < end ? (int32_t)(pt[q] >> 8) : INT32_MAX;
+      if (b + q < cur) continue;
+      const int32_t t = b + q < end ? (int32_t)(pt[q] >> 8) : INT32_MAX;
       go = go && t < ce;
       if (go) atomicAdd(&bkt[t - cs], 1u);
     }
     if (!go) return;
-    cur += WALK_BATCH;
   }
 }
 
@@ -210,37 +211,37 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
                                             int32_t cs, int32_t ce, int il, float x,
                                             uint32_t* bfill, int2* __restrict__ cprod,
                                             unsigned int& n_prod, int32_t* t_next) {
-  for (;;) {
-    uint32_t pt[WALK_BATCH];
-    int2 e[WALK_BATCH];
-#pragma unroll
-    for (int q = 0; q < WALK_BATCH; ++q) {
-      const int c = cur + q < end ? cur + q : end - 1;
-      pt[q] = pos[c];
-      e[q] = ent[c];
-    }
+  const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
+  const int4* ent4 = reinterpret_cast<const int4*>(ent);  // two {row, value bits} per word
+  int consumed = 0;
+  for (int b = cur & ~3;; b += WALK_BATCH) {
+    const uint4 w0 = pos4[b >> 2], w1 = pos4[(b >> 2) + 1];
+    const int4 v0 = ent4[b >> 1], v1 = ent4[(b >> 1) + 1], v2 = ent4[(b >> 1) + 2],
+               v3 = ent4[(b >> 1) + 3];
+    const uint32_t pt[WALK_BATCH] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    const int32_t ev[WALK_BATCH] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w, v3.y, v3.w};
     bool go = true;
     int used = 0;
     int32_t stop_t = INT32_MAX;
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
-      const bool in_list = cur + q < end;
-      const int32_t t = in_list ? (int32_t)(pt[q] >> 8) : INT32_MAX;
+      if (b + q < cur) continue;
+      const int32_t t = b + q < end ? (int32_t)(pt[q] >> 8) : INT32_MAX;
       const bool was = go;
       go = go && t < ce;
       if (go) {
         const uint32_t slot = atomicAdd(&bfill[t - cs], 1u);
-        cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * __int_as_float(e[q].y)));
+        cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * __int_as_float(ev[q])));
         ++used;
       } else if (was) {
         stop_t = t;
       }
     }
-    cur += used;
-    n_prod += (unsigned)used;
+    consumed += used;
     if (!go) {
+      n_prod += (unsigned)consumed;
       *t_next = stop_t;
-      return cur;
+      return cur + consumed;
     }
   }
 }
