@@ -39,7 +39,7 @@ constexpr int CAP_S = 8192;                  // cursor-tier records of a band (b
 constexpr int S_LIST_MAX = 8192;             // longest list sorted in LDS in one piece
 constexpr int MERGE_TILE = 2048;             // outputs per workgroup of a merge pass
 constexpr int CH_TILES = 1024;               // J tiles per cursor-bucket chunk
-constexpr int WALK_PAD = 8;                  // padding entries after the cursor-tier lists
+constexpr int WALK_PAD = 16;                 // padding entries after the cursor-tier lists
 
 // Dense MFMA head: the most frequent tail features, stored per row as a
 // 256-byte record and contracted on the matrix cores.

@@ -178,20 +178,26 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, int tl, uint32_t* w
 
 // Cursor walks over one record's list from position cur, visiting the entries
 // whose J tile lies below ce (lists are row-sorted, so those come first).
-// Entries are read as aligned 16-byte words, eight per step from the multiple
-// of four at or below cur: each lane's step touches one or two cache lines
-// instead of eight separate words.  Entries before cur are skipped (first step
-// only), entries at or after end stop the walk; the list arrays are padded by
-// eight entries, so a step never reads past them.  Callers guarantee cur < end.
-constexpr int WALK_BATCH = 8;
-static_assert(WALK_PAD >= WALK_BATCH, "a walk step reads up to seven entries past end - 1");
+// Entries are read as aligned 16-byte words, WALK_BATCH per step from the
+// multiple of four at or below cur: a lane's step touches a few cache lines
+// instead of WALK_BATCH separate words.  Entries before cur are skipped (first
+// step only), entries at or after end stop the walk; the list arrays are padded
+// by WALK_PAD entries, so a step never reads past them.  Callers guarantee
+// cur < end.
+constexpr int WALK_BATCH = 16;
+static_assert(WALK_BATCH % 4 == 0 && WALK_PAD >= WALK_BATCH,
+              "a walk step reads up to WALK_BATCH - 1 entries past end - 1");
 
 __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
                                            int32_t cs, int32_t ce, uint32_t* bkt) {
   const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
   for (int b = cur & ~3;; b += WALK_BATCH) {
-    const uint4 w0 = pos4[b >> 2], w1 = pos4[(b >> 2) + 1];
-    const uint32_t pt[WALK_BATCH] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    uint32_t pt[WALK_BATCH];
+#pragma unroll
+    for (int w = 0; w < WALK_BATCH / 4; ++w) {
+      const uint4 x = pos4[(b >> 2) + w];
+      pt[4 * w] = x.x; pt[4 * w + 1] = x.y; pt[4 * w + 2] = x.z; pt[4 * w + 3] = x.w;
+    }
     bool go = true;
 #pragma unroll
     for (int q = 0; q < WALK_BATCH; ++q) {
@@ -215,11 +221,19 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
   const int4* ent4 = reinterpret_cast<const int4*>(ent);  // two {row, value bits} per word
   int consumed = 0;
   for (int b = cur & ~3;; b += WALK_BATCH) {
-    const uint4 w0 = pos4[b >> 2], w1 = pos4[(b >> 2) + 1];
-    const int4 v0 = ent4[b >> 1], v1 = ent4[(b >> 1) + 1], v2 = ent4[(b >> 1) + 2],
-               v3 = ent4[(b >> 1) + 3];
-    const uint32_t pt[WALK_BATCH] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-    const int32_t ev[WALK_BATCH] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w, v3.y, v3.w};
+    uint32_t pt[WALK_BATCH];
+    int32_t ev[WALK_BATCH];
+#pragma unroll
+    for (int w = 0; w < WALK_BATCH / 4; ++w) {
+      const uint4 x = pos4[(b >> 2) + w];
+      pt[4 * w] = x.x; pt[4 * w + 1] = x.y; pt[4 * w + 2] = x.z; pt[4 * w + 3] = x.w;
+    }
+#pragma unroll
+    for (int w = 0; w < WALK_BATCH / 2; ++w) {
+      const int4 x = ent4[(b >> 1) + w];
+      ev[2 * w] = x.y;
+      ev[2 * w + 1] = x.w;
+    }
     bool go = true;
     int used = 0;
     int32_t stop_t = INT32_MAX;
