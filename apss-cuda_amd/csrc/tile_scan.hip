@@ -53,6 +53,7 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 // are issued into registers before the current tile's product phase and
 // written to LDS after it, so their latency hides behind the products.
 constexpr int OBUF_CAP = 512;               // LDS output buffer (pairs)
+constexpr int BBUF_CAP = 256;               // LDS buffer of error-band pairs
 static_assert(KD_MAX < 65536 && CAP_J < 65536, "16-bit segment starts");
 
 // Per-mode sizes: bitmap-tier entries per thread of a band and of a tile.
@@ -260,20 +261,60 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
   }
 }
 
-// Copies the LDS output buffer to HBM at one reserved range and empties it.
-// Called by the whole workgroup with obuf_n stable (after a barrier); obuf_n
-// may exceed the capacity (those pairs went to HBM directly).
-__device__ __forceinline__ void flush_obuf(const ScanArgs& a, int tl, const apss_pair* obuf,
-                                           unsigned int* obuf_n, unsigned long long* obuf_base) {
-  const unsigned int nbuf = *obuf_n;
-  const unsigned int nf = nbuf < OBUF_CAP ? nbuf : OBUF_CAP;
-  if (tl == 0) *obuf_base = atomicAdd(a.out_cnt, (unsigned long long)nf);
+// LDS staging of the workgroup's results: emitted pairs and error-band pairs
+// collect here (one LDS atomic per wave and round) and move to HBM in one
+// reserved range per buffer, so the global counters see one atomic per flush.
+struct StageBufs {
+  apss_pair out[OBUF_CAP];
+  unsigned long long band[BBUF_CAP];
+  unsigned int n_out, n_band;          // may exceed the capacity: the excess went to HBM directly
+  unsigned long long base_out, base_band;
+};
+
+// Copies both buffers to HBM and empties them.  Called by the whole workgroup
+// with the counts stable (after a barrier).
+__device__ __forceinline__ void flush_stage(const ScanArgs& a, int tl, StageBufs& sb) {
+  const unsigned int no = sb.n_out < OBUF_CAP ? sb.n_out : OBUF_CAP;
+  const unsigned int nb = sb.n_band < BBUF_CAP ? sb.n_band : BBUF_CAP;
+  if (tl == 0) {
+    sb.base_out = no ? atomicAdd(a.out_cnt, (unsigned long long)no) : 0ull;
+    sb.base_band = nb ? atomicAdd(a.band_cnt, (unsigned long long)nb) : 0ull;
+  }
   lds_barrier();
-  const unsigned long long ob = *obuf_base;
-  for (unsigned int k = tl; k < nf; k += SCAN_THREADS)
-    if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = obuf[k];
+  const unsigned long long ob = sb.base_out, bb = sb.base_band;
+  for (unsigned int k = tl; k < no; k += SCAN_THREADS)
+    if (ob + k < (unsigned long long)a.out_cap) a.out[ob + k] = sb.out[k];
+  for (unsigned int k = tl; k < nb; k += SCAN_THREADS)
+    if (bb + k < (unsigned long long)a.band_cap) a.band[bb + k] = sb.band[k];
   lds_barrier();
-  if (tl == 0) *obuf_n = 0;
+  if (tl == 0) {
+    sb.n_out = 0;
+    sb.n_band = 0;
+  }
+}
+
+// Appends this lane's record (when want) to an LDS staging buffer with one
+// LDS atomic per wave; the part of a reservation beyond the capacity goes
+// straight to HBM through the global counter.
+template <class T>
+__device__ __forceinline__ void stage_append(bool want, const T& rec, T* buf, unsigned int cap,
+                                             unsigned int* n, unsigned long long* counter,
+                                             T* dst, long long dst_cap) {
+  const unsigned long long m = __ballot(want);
+  if (!m) return;
+  const int tot = __popcll(m);
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned int lb = 0;
+  if (lane_id() == leader) lb = atomicAdd(n, (unsigned)tot);
+  lb = __shfl(lb, leader);
+  const unsigned int pos = lb + (unsigned)lane_rank(m);
+  if (want && pos < cap) buf[pos] = rec;
+  if (lb + tot > cap) {
+    int t2;
+    const bool spill = want && pos >= cap;
+    const long long slot = wave_reserve(counter, spill, &t2);
+    if (spill && slot < dst_cap) dst[slot] = rec;
+  }
 }
 
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
@@ -287,9 +328,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ uint32_t bkt[CH_TILES + 1];             // cursor-product bucket starts of the chunk
   __shared__ uint32_t bfill[CH_TILES];               // bucket fill pointers while scattering
   __shared__ uint32_t wsum[SCAN_THREADS / 64 + 1];
-  __shared__ apss_pair obuf[OBUF_CAP];               // emitted pairs waiting for a flush
-  __shared__ unsigned int obuf_n;
-  __shared__ unsigned long long obuf_base;
+  __shared__ StageBufs stage;                        // results waiting for a flush
   // per-row values of tile J in two buffers (by J % 2): a wave that settles the
   // cells of tile J may still read them while the next tile is written
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[2][TILE_ROWS], hnJ[2][TILE_ROWS];
@@ -314,7 +353,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // the tile is cleared cell by cell by the threshold pass, so once is enough
   for (int c = tid0; c < TILE_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
-    acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0; obuf_n = 0;
+    acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0;
+    stage.n_out = 0;
+    stage.n_band = 0;
   }
   unsigned int my_products = 0, my_visits = 0;
 
@@ -486,10 +527,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
 
     lds_barrier();
 
-    // flush the output buffer once half full: every append of the earlier tile
-    // pairs is complete here (the next appends come after the MFMA barrier,
-    // which also orders the reset of obuf_n before them)
-    if (obuf_n >= OBUF_CAP / 2) flush_obuf(a, tl, obuf, &obuf_n, &obuf_base);
+    // flush the staging buffers once one is half full: every append of the
+    // earlier tile pairs is complete here (the next appends come after the
+    // MFMA barrier, which also orders the reset of the counts before them)
+    if (stage.n_out >= OBUF_CAP / 2 || stage.n_band >= BBUF_CAP / 2) flush_stage(a, tl, stage);
 
     // this thread's first cursor product of tile J: its load completes behind
     // the bitmap-tier products.  It is issued before the prefetch of the next
@@ -728,41 +769,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           to_band = true;
         }
       }
-      // emitted pairs go to the workgroup's LDS buffer (one LDS atomic per
-      // wave); a wave that finds the buffer full reserves in HBM directly
-      const unsigned long long me = __ballot(emit);
-      if (me) {
-        const int tot = __popcll(me);
-        const int leader = __ffsll((long long)me) - 1;
-        unsigned int lb = 0;
-        if (lane_id() == leader) lb = atomicAdd(&obuf_n, (unsigned)tot);
-        lb = __shfl(lb, leader);
-        apss_pair pr;
-        pr.i = (uint32_t)(i0 + il + a.i_off);
-        pr.j = (uint32_t)(j0 + jl + a.j_off);
-        pr.score = (float)sd;
-        // every buffer slot below OBUF_CAP is written by the wave that reserved
-        // it; the part of a reservation beyond the buffer goes straight to HBM
-        const unsigned int pos = lb + (unsigned)lane_rank(me);
-        if (emit && pos < OBUF_CAP) obuf[pos] = pr;
-        if (lb + tot > OBUF_CAP) {
-          int t2;
-          const bool spill = emit && pos >= OBUF_CAP;
-          long long slot = wave_reserve(a.out_cnt, spill, &t2);
-          if (spill && slot < a.out_cap) a.out[slot] = pr;
-        }
-      }
-      int tot;
-      long long bslot = wave_reserve(a.band_cnt, to_band, &tot);
-      if (to_band && bslot < a.band_cap)
-        a.band[bslot] = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
+      // emitted pairs and error-band pairs go to the LDS staging buffers
+      apss_pair pr;
+      pr.i = (uint32_t)(i0 + il + a.i_off);
+      pr.j = (uint32_t)(j0 + jl + a.j_off);
+      pr.score = (float)sd;
+      stage_append(emit, pr, stage.out, OBUF_CAP, &stage.n_out, a.out_cnt, a.out, a.out_cap);
+      const unsigned long long key = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
+      stage_append(to_band, key, stage.band, BBUF_CAP, &stage.n_band, a.band_cnt, a.band, a.band_cap);
     }
   }
   __syncthreads();
   }  // band queue
 
-  // pairs still in the output buffer (the barrier above ends every append)
-  if (obuf_n > 0) flush_obuf(a, tid0, obuf, &obuf_n, &obuf_base);
+  // pairs still in the staging buffers (the barrier above ends every append)
+  if (stage.n_out > 0 || stage.n_band > 0) flush_stage(a, tid0, stage);
   atomicAdd(&acc_products, (unsigned long long)my_products);
   atomicAdd(&acc_visits, (unsigned long long)my_visits);
   __syncthreads();
