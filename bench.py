@@ -40,6 +40,9 @@ Legs printed on the same JSON line (each can be switched off):
   cpu       the reference algorithm restated for the CPU (oracle/), timed on
             the host cores on a stratified sample of query rows, each joined
             with every later row (pair-space throughput).
+  cpu_serial  the reference's serial inverted-index search (main.cu:104-141,
+            oracle/), one host thread, on a seeded random sample of query rows
+            (a second CPU data point).
 """
 import argparse
 import json

@@ -235,3 +235,31 @@ def test_text_file_to_pair_file(tmp_path):
     apss.write_pairs(b, e)
     assert len(got) > 0
     assert apss.compare_pair_files(a, b) == (0, 0)
+
+
+def test_cursor_walk_alignment():
+    # The cursor walks read list entries as aligned 16-byte words from the
+    # multiple of four at or below the cursor and mask the rest.  Lists of
+    # lengths around multiples of 4 and 16 start at every offset of the global
+    # list array, and with 70000 rows (1094 tiles, more than one 1024-tile
+    # bucket chunk) walks also stop inside a step at a chunk end.  Each row has
+    # six features out of 4000; the fixed-length lists carry dominant values,
+    # so the rows sharing one form the result pairs.
+    rng = np.random.default_rng(5)
+    n, d = 70000, 4100
+    lengths = [1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 31, 32, 33, 47, 48, 49, 63, 64, 65, 255, 256, 257,
+               1023, 1024, 1025]
+    rows = [dict() for _ in range(n)]
+    for r in range(n):
+        for f in rng.choice(np.arange(100, d), size=6, replace=False):
+            rows[r][int(f)] = float(rng.uniform(0.05, 0.3))
+    for f, L in enumerate(lengths, start=10):
+        for r in rng.choice(n, size=L, replace=False):
+            rows[r][f] = float(rng.uniform(2.0, 4.0))
+    for r in range(n):
+        v = np.array(list(rows[r].values()))
+        s = float(np.linalg.norm(v))
+        rows[r] = {k: float(np.float32(x / s)) for k, x in rows[r].items()}
+    rp, col, val, d = tiny_matrix(rows, d)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.35, 0, dense_features=1, mfma_head=-1)
+    assert info["cursor_products"] > 0 and len(exp) > 100000
