@@ -577,11 +577,22 @@ void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
 Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
                    int32_t dm, const std::vector<int64_t>& bounds) {
   Built b;
+  PhaseTrace ph;
+  // with APSS_TRACE set, each step is synchronised so the trace shows its device time
+  auto step = [&](const char* what) {
+    if (!trace_on()) return;
+    APSS_HIP(hipStreamSynchronize(st));
+    ph.mark(what);
+  };
   b.tr = build_tiers(ar, st, s, p, self, kd_cap, dm);
+  step("index: tiers + cursor lists");
   b.bands = build_tiles(ar, st, s, b.tr, self, BAND_ROWS, cap_i_for(b.tr.dm), CAP_S, true, bounds);
+  step("index: band blocks");
   b.tiles = build_tiles(ar, st, s, b.tr, self, TILE_ROWS, cap_j_for(b.tr.dm), INT32_MAX, false, bounds);
+  step("index: tile blocks");
   set_cursor_tiles(ar, st, b.tr, b.tiles);
   build_dense(ar, st, s, b.tr);
+  step("index: cursor tiles + dense");
   return b;
 }
 
