@@ -188,7 +188,6 @@ def test_reference_signature_shim_counts_pairs():
     rp, col, val, d = small_rows(2400)
     # eight partitions, the last one smaller (as in main.cu:180-187)
     cuts = [0, 300, 600, 900, 1200, 1500, 1800, 2100, 2400]
-    parts = [subset(*[x for x in (rp, col, val)], 0)] * 0
     parts = []
     for a, b in zip(cuts[:-1], cuts[1:]):
         prp = (rp[a:b + 1] - rp[a]).astype(np.int64)
@@ -196,3 +195,19 @@ def test_reference_signature_shim_counts_pairs():
     count = apss.similarity_join_device(parts)
     exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
     assert count == len(exp)
+
+
+def test_text_file_through_reference_shim(tmp_path):
+    # A main.cu-style caller: rows read from the reference's text format, cut
+    # into eight partitions (the last one smaller), handed to the 70-argument
+    # shim as SELL-32 slices; the count equals the oracle's at t = 0.3f.
+    rp, col, val, d = small_rows(2000)
+    path = str(tmp_path / "rows.txt")
+    apss.save_text(path, rp, col, val, d)
+    rp, col, val, d = apss.load_text(path)
+    cuts = [0, 256, 512, 768, 1024, 1280, 1536, 1792, 2000]
+    parts = [((rp[a:b + 1] - rp[a]).astype(np.int64), col[rp[a]:rp[b]].copy(), val[rp[a]:rp[b]].copy())
+             for a, b in zip(cuts[:-1], cuts[1:])]
+    count = apss.similarity_join_device(parts)
+    exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
+    assert count == len(exp) > 0
