@@ -9,24 +9,30 @@
 // (:441-506, :854-915) dot the head of every survivor against a dense LDS copy
 // of one row and test >= t, with the emission left commented out.
 //
-// Here one workgroup owns one band: a tile I of 128 rows.  It sweeps its
-// partner tiles J (J >= I for the self-join) and for each keeps the 128x128
-// score tile in LDS, so the n^2 matrix never exists in HBM:
+// Here a persistent workgroup (one per CU, 1024 threads) takes work items from
+// a queue; an item is a band I (up to 128 rows) against a range of its partner
+// tiles J (up to 64 rows each; J >= I's own tiles for the self-join).  For each
+// tile pair it keeps the 128 x 64 score tile in LDS as int32 fixed point, so
+// the n^2 matrix never exists in HBM:
 //
-//   1. cursor tier: every short-list entry of band I keeps a cursor into its
-//      (row-sorted) inverted list; the cursors that point into J advance and
-//      add their products to the tile;
-//   2. bitmap tier: AND of the two tiles' presence bitmaps gives the common
-//      long-list features; each contributes the outer product of its two
-//      tile segments (wave per feature when large, lane per feature when
-//      small);
-//   3. head (p > 0): the prefix-norm bound partial + |x_head||y_head| >= t
+//   1. cursor tier (rare features): every band record keeps a cursor into its
+//      row-sorted inverted list; per chunk of up to 1024 tiles the records
+//      walk their lists (count, then scatter) and bucket their products by J
+//      tile in a per-workgroup HBM region; each tile pair adds its bucket;
+//   2. bitmap tier (frequent features): every band entry reads its feature's
+//      segment in the J tile block (segment starts by bitmap index) and adds
+//      the products with integer LDS atomics;
+//   3. dense head (most frequent features): the band's and the tile's 256-byte
+//      head records are contracted on the matrix cores, as exact int8 limbs or
+//      in BF16;
+//   4. head (p > 0): the prefix-norm bound partial + |x_head||y_head| >= t
 //      (rigorous, with the FP32 error bound) selects the cells whose
 //      forward-list head dot is computed; others keep their partial score;
-//   4. threshold: every valid cell is compared with t using its rigorous FP32
-//      error bound; certain hits are appended to the output with one atomic
-//      per wave (ballot + mbcnt ranks), cells within the bound go to the band
-//      list that the FP64 rescore kernel settles exactly.
+//   5. threshold: each lane reads and clears its eight cells and settles the
+//      ones that can reach t against their rigorous error bound: certain hits
+//      go to the output, cells within the bound to the band list that the FP64
+//      rescore kernel settles exactly.  Both go through LDS staging buffers
+//      and reach HBM with one global atomic per buffer per flush.
 #include "apss_internal.h"
 
 namespace apss {
