@@ -128,7 +128,8 @@ struct TileSide {
 };
 
 // Cursor-tier inverted lists of the corpus side: {row, float bits of value},
-// ascending row inside each list.
+// ascending row inside each list.  ent and pos hold WALK_PAD entries past the
+// last list, so the band kernel's walks can read aligned 16-byte words.
 struct CursorLists {
   const int64_t* ptr = nullptr;   // d+1
   const int2* ent = nullptr;
