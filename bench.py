@@ -355,7 +355,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the tile-pair space shards across GPUs by bands; see DESIGN.md)", file=sys.stderr)
+              f"(the tile-pair space would shard across GPUs by work items)", file=sys.stderr)
         sys.exit(2)
     skip = set(ALL_LEGS) if args.only_main else {s for s in args.skip.split(",") if s}
 

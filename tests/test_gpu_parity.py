@@ -1,8 +1,8 @@
 """Parity of the HIP path (through the C-ABI) with the FP64 oracle.
 
-Exact pair-set equality and scores within 1e-6 relative (the tolerance of the
-north-star contract), on seeded synthetic matrices at sizes the oracle
-finishes in seconds, plus the edge cases of the method.
+Exact pair-set equality and scores within 1e-6 relative (the library's score
+tolerance), on seeded synthetic matrices at sizes the oracle finishes in
+seconds, plus the edge cases of the method.
 """
 import numpy as np
 import pytest
