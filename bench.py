@@ -354,7 +354,7 @@ def main():
     ap.add_argument("--serving-max", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the tile-pair space would shard across GPUs by work items)", file=sys.stderr)
         sys.exit(2)
     skip = set(ALL_LEGS) if args.only_main else {s for s in args.skip.split(",") if s}
