@@ -72,9 +72,8 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
     size_t m = (size_t)std::min<int64_t>((int64_t)chunk, r.count - k);
     emit(r.pairs + k, m, user);
   }
-  std::free(r.pairs);
+  apss::host_pairs_free(r.pairs);
   r.pairs = nullptr;
-  r.count = r.count;
   if (out) *out = r;
   return APSS_OK;
 }
@@ -88,7 +87,7 @@ void apss_result_free(apss_result* r) {
   if (!r) return;
   if (r->pairs) {
     if (r->on_device) (void)hipFree(r->pairs);
-    else std::free(r->pairs);
+    else apss::host_pairs_free(r->pairs);
   }
   r->pairs = nullptr;
   r->count = 0;

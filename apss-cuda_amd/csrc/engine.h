@@ -258,6 +258,15 @@ struct StreamScope {
 
 // Sorting and copying helpers.
 void sort_pairs(apss_pair* p, int64_t n);
+
+// Host result blocks.  Results of at least 1 MiB live in pinned memory from a
+// process-wide pool (blocks are reused across results, so device-to-host
+// copies run at DMA speed into pages that are already resident); smaller ones
+// come from malloc.  Every host result (apss_result.pairs with on_device = 0)
+// is allocated and released through these.
+apss_pair* host_pairs_alloc(int64_t count);                       // room for max(count, 1) pairs
+apss_pair* host_pairs_grow(apss_pair* p, int64_t used, int64_t count);  // keeps the first `used`
+void host_pairs_free(apss_pair* p);
 inline int blocks_for(int64_t n, int threads, int cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;

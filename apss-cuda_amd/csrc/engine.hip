@@ -13,7 +13,10 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "engine.h"
 
@@ -974,9 +977,98 @@ void sort_pairs(apss_pair* p, int64_t n) {
   });
 }
 
-struct DevicePairs {  // result.internal_ for device-resident output
-  void* ptr;
+// ---------------------------------------------------------------------------
+// host result blocks
+
+namespace {
+constexpr size_t PINNED_MIN = (size_t)1 << 20;      // smaller results use malloc
+constexpr size_t PINNED_POOL_MAX = (size_t)8 << 30;  // pinned bytes kept for reuse
+
+struct PinnedPool {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;  // pinned blocks handed out, by size
+  std::multimap<size_t, void*> idle;        // pinned blocks ready for reuse
+  size_t idle_bytes = 0;
 };
+PinnedPool& pinned_pool() {
+  static PinnedPool* pp = new PinnedPool;  // outlives every static destructor
+  return *pp;
+}
+// Sizes are rounded up to 1/8 steps of a power of two, so blocks get reused.
+size_t pinned_size(size_t bytes) {
+  size_t pw = PINNED_MIN;
+  while (pw * 2 <= bytes) pw *= 2;
+  const size_t step = pw / 8;
+  return (bytes + step - 1) / step * step;
+}
+}  // namespace
+
+apss_pair* host_pairs_alloc(int64_t count) {
+  const size_t bytes = sizeof(apss_pair) * (size_t)std::max<int64_t>(count, 1);
+  if (bytes >= PINNED_MIN) {
+    PinnedPool& pp = pinned_pool();
+    const size_t want = pinned_size(bytes);
+    {
+      std::lock_guard<std::mutex> lk(pp.mu);
+      auto it = pp.idle.lower_bound(want);
+      if (it != pp.idle.end() && it->first <= 2 * want) {
+        void* q = it->second;
+        pp.live[q] = it->first;
+        pp.idle_bytes -= it->first;
+        pp.idle.erase(it);
+        return (apss_pair*)q;
+      }
+    }
+    void* q = nullptr;
+    if (hipHostMalloc(&q, want, hipHostMallocDefault) == hipSuccess && q) {
+      std::lock_guard<std::mutex> lk(pp.mu);
+      pp.live[q] = want;
+      return (apss_pair*)q;
+    }
+    (void)hipGetLastError();  // no pinned memory: fall back to pageable
+  }
+  apss_pair* q = (apss_pair*)std::malloc(bytes);
+  if (!q) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+  return q;
+}
+
+apss_pair* host_pairs_grow(apss_pair* p, int64_t used, int64_t count) {
+  if (!p) return host_pairs_alloc(count);
+  size_t have = 0;
+  {
+    PinnedPool& pp = pinned_pool();
+    std::lock_guard<std::mutex> lk(pp.mu);
+    auto it = pp.live.find(p);
+    if (it != pp.live.end()) have = it->second;
+  }
+  const size_t bytes = sizeof(apss_pair) * (size_t)std::max<int64_t>(count, 1);
+  if (have >= bytes) return p;
+  apss_pair* q = host_pairs_alloc(count);
+  if (used > 0) std::memcpy(q, p, sizeof(apss_pair) * (size_t)used);
+  host_pairs_free(p);
+  return q;
+}
+
+void host_pairs_free(apss_pair* p) {
+  if (!p) return;
+  PinnedPool& pp = pinned_pool();
+  std::unique_lock<std::mutex> lk(pp.mu);
+  auto it = pp.live.find(p);
+  if (it == pp.live.end()) {
+    lk.unlock();
+    std::free(p);
+    return;
+  }
+  const size_t sz = it->second;
+  pp.live.erase(it);
+  if (pp.idle_bytes + sz <= PINNED_POOL_MAX) {
+    pp.idle.emplace(sz, (void*)p);
+    pp.idle_bytes += sz;
+    return;
+  }
+  lk.unlock();
+  (void)hipHostFree(p);
+}
 
 int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out) {
   apss_opts def;
@@ -1055,8 +1147,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
     out->pairs = (apss_pair*)ar.detach(so.pairs);
     out->on_device = 1;
   } else {
-    apss_pair* h = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(so.count, 1));
-    if (!h) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+    apss_pair* h = host_pairs_alloc(so.count);
     if (so.count)
       APSS_HIP(hipMemcpyAsync(h, so.pairs, sizeof(apss_pair) * so.count, hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
@@ -1065,7 +1156,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
       apss_pair* d = so.pairs;
       APSS_HIP(hipMemcpyAsync(d, h, sizeof(apss_pair) * so.count, hipMemcpyHostToDevice, st));
       APSS_HIP(hipStreamSynchronize(st));
-      std::free(h);
+      host_pairs_free(h);
       out->pairs = (apss_pair*)ar.detach(d);
       out->on_device = 1;
     } else {
@@ -1099,8 +1190,7 @@ StreamScope::~StreamScope() {
 
 void finish_result(hipStream_t st, const apss_pair* dpairs, int64_t count, bool sort,
                    apss_result* out) {
-  apss_pair* h = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(count, 1));
-  if (!h) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+  apss_pair* h = host_pairs_alloc(count);
   if (count)
     APSS_HIP(hipMemcpyAsync(h, dpairs, sizeof(apss_pair) * count, hipMemcpyDeviceToHost, st));
   APSS_HIP(hipStreamSynchronize(st));
@@ -1247,8 +1337,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     ++cnt[k];
   }
   for (int32_t k = 0; k < count; ++k) {
-    out[k].pairs = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(cnt[k], 1));
-    if (!out[k].pairs) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+    out[k].pairs = host_pairs_alloc(cnt[k]);
     out[k].count = 0;
   }
   for (int64_t q = 0; q < so.count; ++q) {

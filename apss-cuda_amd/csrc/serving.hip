@@ -125,21 +125,19 @@ static std::unique_ptr<Segment> build_segment(HostCSR&& h, int32_t p, const apss
   return sg;
 }
 
-// Growable host pair buffer (malloc, no zero fill); release() hands the block
-// to a result, which apss_result_free releases with free().
+// Growable host pair buffer (a host result block, see host_pairs_alloc);
+// release() hands the block to a result, which apss_result_free releases.
 struct PairBuf {
   apss_pair* p = nullptr;
   int64_t n = 0, cap = 0;
   PairBuf() = default;
   PairBuf(const PairBuf&) = delete;
   PairBuf& operator=(const PairBuf&) = delete;
-  ~PairBuf() { std::free(p); }
+  ~PairBuf() { host_pairs_free(p); }
   apss_pair* grow(int64_t extra) {  // room for `extra` more pairs; returns where they go
     if (n + extra > cap) {
       const int64_t c = std::max<int64_t>(n + extra, std::max<int64_t>(1, cap + cap / 2));
-      void* q = std::realloc(p, sizeof(apss_pair) * (size_t)c);
-      if (!q) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
-      p = (apss_pair*)q;
+      p = host_pairs_grow(p, n, c);
       cap = c;
     }
     apss_pair* at = p + n;
@@ -147,8 +145,7 @@ struct PairBuf {
     return at;
   }
   apss_pair* release() {
-    if (!p) p = (apss_pair*)std::malloc(sizeof(apss_pair));
-    if (!p) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
+    if (!p) p = host_pairs_alloc(1);
     apss_pair* q = p;
     p = nullptr;
     n = cap = 0;
@@ -359,8 +356,7 @@ static void worker_loop(Server* s, int slot) {
         for (int64_t q = 0; q < pairs.n; ++q) ++cnt[(size_t)req_of[pairs.p[q].i]];
         for (size_t m = 0; m < live.size(); ++m) {
           outn[live[m]] = cnt[m];
-          outp[live[m]] = (apss_pair*)std::malloc(sizeof(apss_pair) * (size_t)std::max<int64_t>(cnt[m], 1));
-          if (!outp[live[m]]) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+          outp[live[m]] = host_pairs_alloc(cnt[m]);
         }
         for (int64_t q = 0; q < pairs.n; ++q) {
           apss_pair x = pairs.p[q];
@@ -371,8 +367,7 @@ static void worker_loop(Server* s, int slot) {
       }
       for (size_t k = 0; k < batch.size(); ++k)  // requests without rows (or pairs)
         if (rc == APSS_OK && !outp[k]) {
-          outp[k] = (apss_pair*)std::malloc(sizeof(apss_pair));
-          if (!outp[k]) throw Error{APSS_E_NOMEM, "host allocation of a result failed"};
+          outp[k] = host_pairs_alloc(1);
         }
     } catch (const Error& e) {
       rc = e.code;
@@ -387,7 +382,7 @@ static void worker_loop(Server* s, int slot) {
         r->res = stats;
         r->res.count = rc == APSS_OK ? outn[k] : 0;
         r->res.pairs = rc == APSS_OK ? outp[k] : nullptr;
-        if (rc != APSS_OK && outp[k]) std::free(outp[k]);
+        if (rc != APSS_OK && outp[k]) host_pairs_free(outp[k]);
         r->res.on_device = 0;
         r->res.p_used = s->idx->p;
         r->state = Request::DONE;
@@ -593,7 +588,7 @@ int apss_server_wait(apss_server* s, int64_t ticket, apss_result* out, double* l
     *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   const int rc = r->rc;
   if (rc != APSS_OK && out->pairs) {
-    std::free(out->pairs);
+    host_pairs_free(out->pairs);
     out->pairs = nullptr;
     out->count = 0;
   }
@@ -633,7 +628,7 @@ int apss_server_destroy(apss_server* s) {
   S.cv_done.notify_all();
   for (auto& w : S.workers) w.join();
   for (auto& kv : S.reqs)
-    if (kv.second->res.pairs) std::free(kv.second->res.pairs);
+    if (kv.second->res.pairs) host_pairs_free(kv.second->res.pairs);
   delete s;
   return APSS_OK;
 }
