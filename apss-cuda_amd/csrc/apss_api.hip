@@ -63,19 +63,35 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   apss_opts local;
   apss_opts_init(&local);
   if (o) local = *o;
-  local.output_on_device = 0;
+  // the pairs stay on the device and reach the host in chunks through one
+  // pinned staging block, so the host never holds the whole result
+  local.output_on_device = 1;
   apss_result r;
   int rc = apss_run(X, t, p, &local, &r);
   if (rc != APSS_OK) return rc;
-  const size_t chunk = (size_t)1 << 20;
-  for (int64_t k = 0; k < r.count; k += (int64_t)chunk) {
-    size_t m = (size_t)std::min<int64_t>((int64_t)chunk, r.count - k);
-    emit(r.pairs + k, m, user);
-  }
-  apss::host_pairs_free(r.pairs);
-  r.pairs = nullptr;
-  if (out) *out = r;
-  return APSS_OK;
+  return guarded([&] {
+    const int64_t chunk = (int64_t)1 << 20;
+    apss_pair* stage = host_pairs_alloc(std::min<int64_t>(chunk, std::max<int64_t>(r.count, 1)));
+    struct Release {
+      apss_pair* s;
+      apss_result* r;
+      ~Release() {
+        host_pairs_free(s);
+        apss_result_free(r);
+      }
+    } rel{stage, &r};
+    for (int64_t k = 0; k < r.count; k += chunk) {
+      const int64_t m = std::min<int64_t>(chunk, r.count - k);
+      APSS_HIP(hipMemcpy(stage, r.pairs + k, sizeof(apss_pair) * (size_t)m, hipMemcpyDeviceToHost));
+      emit(stage, (size_t)m, user);
+    }
+    if (out) {
+      *out = r;
+      out->pairs = nullptr;
+      out->on_device = 0;
+    }
+    return APSS_OK;
+  });
 }
 
 int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,

@@ -259,9 +259,9 @@ struct StreamScope {
 // Sorting and copying helpers.
 void sort_pairs(apss_pair* p, int64_t n);
 
-// Host result blocks.  Results of at least 1 MiB live in pinned memory from a
+// Host result blocks.  Results of 1 MiB to 2 GiB live in pinned memory from a
 // process-wide pool (blocks are reused across results, so device-to-host
-// copies run at DMA speed into pages that are already resident); smaller ones
+// copies run at DMA speed into pages that are already resident); the others
 // come from malloc.  Every host result (apss_result.pairs with on_device = 0)
 // is allocated and released through these.
 apss_pair* host_pairs_alloc(int64_t count);                       // room for max(count, 1) pairs

@@ -982,6 +982,7 @@ void sort_pairs(apss_pair* p, int64_t n) {
 
 namespace {
 constexpr size_t PINNED_MIN = (size_t)1 << 20;      // smaller results use malloc
+constexpr size_t PINNED_MAX = (size_t)2 << 30;      // larger ones too (pinning is slow)
 constexpr size_t PINNED_POOL_MAX = (size_t)8 << 30;  // pinned bytes kept for reuse
 
 struct PinnedPool {
@@ -1005,7 +1006,7 @@ size_t pinned_size(size_t bytes) {
 
 apss_pair* host_pairs_alloc(int64_t count) {
   const size_t bytes = sizeof(apss_pair) * (size_t)std::max<int64_t>(count, 1);
-  if (bytes >= PINNED_MIN) {
+  if (bytes >= PINNED_MIN && bytes <= PINNED_MAX) {
     PinnedPool& pp = pinned_pool();
     const size_t want = pinned_size(bytes);
     {

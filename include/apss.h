@@ -133,8 +133,10 @@ void apss_opts_init(apss_opts* o);
  * Replaces similarity_join_device (similarity_search.h:3-14) for one matrix. */
 int apss_run(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
 
-/* Streaming variant: pairs are handed to emit() in chunks on the calling
- * thread instead of being returned; out (optional) receives the counters. */
+/* Streaming variant: pairs are handed to emit() in chunks (up to 2^20 pairs)
+ * on the calling thread instead of being returned; out (optional) receives the
+ * counters.  The pairs stay on the device until their chunk is copied, so the
+ * host never holds more than one chunk; opts.output_on_device is ignored. */
 int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_emit_fn emit,
                 void* user, apss_result* out);
 

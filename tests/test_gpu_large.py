@@ -60,7 +60,7 @@ def stream(large, t, **opts):
     rp, col, val, d, sample = large
     mask = np.zeros(len(rp) - 1, bool)
     mask[sample] = True
-    acc = {"count": 0, "sum": np.uint64(0), "keys": [], "scores": []}
+    acc = {"count": 0, "sum": 0, "keys": [], "scores": []}
 
     def emit(chunk):
         i = chunk["i"]
@@ -68,7 +68,7 @@ def stream(large, t, **opts):
         assert np.all(i < j)
         acc["count"] += len(chunk)
         h = (i.astype(np.uint64) * K1) ^ (j.astype(np.uint64) * K2)
-        acc["sum"] = np.uint64(acc["sum"] + h.sum(dtype=np.uint64))
+        acc["sum"] = (acc["sum"] + int(h.sum(dtype=np.uint64))) % (1 << 64)  # sum mod 2^64
         m = mask[i] | mask[j]
         if m.any():
             acc["keys"].append(i[m].astype(np.int64) * (1 << 32) + j[m].astype(np.int64))
