@@ -71,15 +71,16 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   if (rc != APSS_OK) return rc;
   return guarded([&] {
     const int64_t chunk = (int64_t)1 << 20;
-    apss_pair* stage = host_pairs_alloc(std::min<int64_t>(chunk, std::max<int64_t>(r.count, 1)));
-    struct Release {
+    struct Release {  // the device result and the staging block, on every exit
       apss_pair* s;
       apss_result* r;
       ~Release() {
         host_pairs_free(s);
         apss_result_free(r);
       }
-    } rel{stage, &r};
+    } rel{nullptr, &r};
+    apss_pair* stage = host_pairs_alloc(std::min<int64_t>(chunk, std::max<int64_t>(r.count, 1)));
+    rel.s = stage;
     for (int64_t k = 0; k < r.count; k += chunk) {
       const int64_t m = std::min<int64_t>(chunk, r.count - k);
       APSS_HIP(hipMemcpy(stage, r.pairs + k, sizeof(apss_pair) * (size_t)m, hipMemcpyDeviceToHost));
