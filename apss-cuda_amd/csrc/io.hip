@@ -244,13 +244,15 @@ int64_t apss_compare_pair_files(const char* a, const char* b, int64_t* only_a, i
   auto read = [](const char* path, std::unordered_set<uint64_t>& s) -> int {
     File f(path, "r");
     if (!f.f) return fail(APSS_E_INVALID, std::string("cannot open ") + path);
-    char buf[256];
-    while (std::fgets(buf, sizeof(buf), f.f)) {
+    char* buf = nullptr;
+    size_t cap = 0;
+    while (getline(&buf, &cap, f.f) > 0) {
       unsigned long i, j;
       if (std::sscanf(buf, "%lu %lu", &i, &j) != 2) continue;  // blank or malformed line
       const uint64_t x = std::min(i, j), y = std::max(i, j);
       s.insert(x << 32 | y);
     }
+    std::free(buf);
     return APSS_OK;
   };
   std::unordered_set<uint64_t> sa, sb;
