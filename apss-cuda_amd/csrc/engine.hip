@@ -471,7 +471,8 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   ts.cuts = cuts;
   ts.tstart = ar.alloc<int64_t>(T + 1);
   APSS_HIP(hipMemcpyAsync(ts.tstart, cuts.data(), sizeof(int64_t) * (T + 1), hipMemcpyHostToDevice, st));
-  if (T >= 0xffff) throw Error{APSS_E_UNSUPPORTED, "more than 65534 tiles"};
+  // cursor-list positions pack the J tile into 24 bits (tile << 8 | row)
+  if (T >= (1 << 24)) throw Error{APSS_E_UNSUPPORTED, "more than 2^24 - 1 tiles"};
   ts.row2tile = ar.alloc<int32_t>(s.n);
   if (T > 0) {
     hipLaunchKernelGGL(k_row2tile, dim3(T), dim3(256), 0, st, ts.tstart, ts.row2tile);

@@ -263,3 +263,25 @@ def test_cursor_walk_alignment():
     rp, col, val, d = tiny_matrix(rows, d)
     got, exp, info, err = run_and_check(rp, col, val, d, 0.35, 0, dense_features=1, mfma_head=-1)
     assert info["cursor_products"] > 0 and len(exp) > 100000
+
+
+def test_more_than_65535_tiles():
+    # A corpus of 4.5e6 rows of one feature each is cut into 70313 tiles of 64
+    # rows; a 100-row query meets every corpus row sharing its feature with
+    # score 1 (the query is one band, so the scan stays small)
+    rng = np.random.default_rng(21)
+    n, d = 4_500_000, 2_000_000
+    rp = np.arange(n + 1, dtype=np.int64)
+    col = rng.integers(0, d, size=n).astype(np.int32)
+    val = np.ones(n, np.float32)
+    ix = apss.Index(rp, col, val, d, p=0)
+    try:
+        qcol = col[rng.choice(n, 100, replace=False)]
+        qrp = np.arange(101, dtype=np.int64)
+        got, info = ix.query(qrp, qcol, np.ones(100, np.float32), 0.99)
+    finally:
+        ix.close()
+    assert info["tile_pairs"] >= 70313
+    expected = {(q, int(r)) for q in range(100) for r in np.flatnonzero(col == qcol[q])}
+    assert {(int(a), int(b)) for a, b in zip(got["i"], got["j"])} == expected
+    assert np.all(got["score"] == np.float32(1.0))
