@@ -20,7 +20,7 @@ Legs printed on the same JSON line (each can be switched off):
   mixed     the mixed-precision configuration (n = d = 5e5, mean 80 nnz/row,
             Zipf(1.2), uniform values, t = 0.5): plain path, exact int8 head,
             BF16 head -- the pair sets must be identical;
-  split     the split p swept over {0, 1024, 4096, 16384} at t = 0.5 and 0.9 on
+  split     the split p swept over {0, 1024, 2048, 4096, 8192, 16384} at t = 0.5 and 0.9 on
             the first 50,000 rows of the large workload (p > 0 runs the
             reference's prefix-norm filter + forward-list head dot); the pair
             counts must not depend on p;
@@ -174,12 +174,15 @@ def leg_mixed(hip, stream):
 
 
 def leg_split(n=50_000):
-    """Split sweep on a slice of the large workload: time and pair count per p."""
+    """Split sweep on a slice of the large workload: time and pair count per p.
+    With p > 0 the ids below p run the reference's prefix-norm filter and
+    forward-list head dot for every surviving pair, whose cost grows with p,
+    so the sweep uses the first 50,000 rows; the pair set must not change."""
     rp, col, val, d = apss.workload("large", n=n)
     out = {"rows": n, "sweep": []}
     for t in (0.5, 0.9):
         counts = set()
-        for p in (0, 1024, 4096, 16384):
+        for p in (0, 1024, 2048, 4096, 8192, 16384):
             _, info = apss.run(rp, col, val, d, t, p, output_on_device=1)
             counts.add(info["count"])
             out["sweep"].append({"t": t, "p": p, "scan_ms": round(info["ms_scan"], 1),
