@@ -173,9 +173,11 @@ int apss_server_destroy(apss_server* s);
  * the same 70 arguments (8 SELL-32 partitions, their head norms, sizes and
  * the three lookup tables).  The SELL-32 slices are converted to one CSR
  * matrix (padding -1 dropped, rows in partition order A..H), then apss_run
- * runs with the reference constants t = 0.3f (similarity_search.cu:18) and
- * p = 4096.  Like the reference it prints a summary and returns nothing; the
- * pair count of the last call is available from apss_compat_last_count(). */
+ * runs at the reference's threshold t = 0.3f (similarity_search.cu:18) with
+ * p = 0 (the pair set does not depend on p; the reference's literal 4096 only
+ * moves work into its slow pair-parallel head dot).  Like the reference it
+ * prints a summary and returns nothing; the pair count of the last call is
+ * available from apss_compat_last_count(). */
 void similarity_join_device(int* dim_A, float* feature_A, int* tile_start_A, int* dim_B,
                             float* feature_B, int* tile_start_B, int* dim_C, float* feature_C,
                             int* tile_start_C, int* dim_D, float* feature_D, int* tile_start_D,
