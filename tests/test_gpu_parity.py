@@ -285,3 +285,40 @@ def test_more_than_65535_tiles():
     expected = {(q, int(r)) for q in range(100) for r in np.flatnonzero(col == qcol[q])}
     assert {(int(a), int(b)) for a, b in zip(got["i"], got["j"])} == expected
     assert np.all(got["score"] == np.float32(1.0))
+
+
+@pytest.mark.parametrize("sort_output", [0, 1])
+def test_device_inputs_and_device_output(sort_output):
+    # The mode the benchmark runs in: the CSR already in device memory and the
+    # pairs left there (copied back here only to check them); with
+    # sort_output the device result is sorted by (i, j).
+    rp, col, val, d = apss.workload("small", n=4000)
+    hip = apss.Hip()
+    X = apss.DeviceCSR(hip, rp, col, val, d)
+    try:
+        r = apss.run_csr(X.csr(), 0.5, 0, output_on_device=1, sort_output=sort_output)
+        try:
+            assert r.on_device == 1
+            got = np.zeros(r.count, apss.PAIR_DTYPE)
+            if r.count:
+                hip.d2h(got, r.pairs)
+        finally:
+            apss.lib().apss_result_free(apss.ctypes.byref(r))
+    finally:
+        X.release()
+    exp = oracle.apss_fp64(rp, col, val, d, 0.5)
+    check_pairs(got, exp)
+    if sort_output:
+        key = got["i"].astype(np.int64) * (1 << 32) + got["j"]
+        assert np.all(np.diff(key) > 0)
+
+
+@pytest.mark.parametrize("mfma_head", [0, -1])
+def test_mixed_workload_subset_exact(mfma_head):
+    # the heavy Zipf(1.2) head of the mixed workload through the exact int8
+    # MFMA head and through the plain sparse path: exact pair set, scores
+    # within 1e-6 relative
+    rp, col, val, d = apss.workload("mixed", n=8000)
+    got, exp, info, err = run_and_check(rp, col, val, d, 0.5, 0, mfma_head=mfma_head)
+    assert info["mfma_features"] == (64 if mfma_head == 0 else 0)
+    assert len(exp) > 10000
