@@ -54,12 +54,16 @@ constexpr int DENSE_I8_FEATURES = 64;
 constexpr int DENSE_BF16_FEATURES = 128;
 // With a dense head the sparse blocks are smaller, and the LDS they free holds
 // the head operands: caps of the bitmap tier per band / tile / row.
+// The int8 head's kernel has a little more LDS left than the BF16 one (whose
+// per-row head norms take some), so its tile cap and bitmap tier are larger.
 constexpr int CAP_I_DENSE = 6144;
-constexpr int CAP_J_DENSE = 3072;
-constexpr int KD_MAX_DENSE = 3072;
+constexpr int CAP_J_I8 = 3424, CAP_J_BF16 = 3072;
 __host__ __device__ constexpr int cap_i_for(int dm) { return dm ? CAP_I_DENSE : CAP_I; }
-__host__ __device__ constexpr int cap_j_for(int dm) { return dm ? CAP_J_DENSE : CAP_J; }
-__host__ __device__ constexpr int kd_max_for(int dm) { return dm ? KD_MAX_DENSE : KD_MAX; }
+__host__ __device__ constexpr int cap_j_for(int dm) {
+  return dm == DENSE_I8 ? CAP_J_I8 : dm == DENSE_BF16 ? CAP_J_BF16 : CAP_J;
+}
+// a row holds at most KD bitmap-tier entries and must fit one tile block
+__host__ __device__ constexpr int kd_max_for(int dm) { return dm ? cap_j_for(dm) : KD_MAX; }
 
 // Feature classes in fmap[] (values >= 0 are bitmap-tier indices).
 constexpr int32_t F_HEAD = -1;    // id < p: head (forward list / MFMA)
