@@ -57,7 +57,7 @@ constexpr int DENSE_BF16_FEATURES = 128;
 // The int8 head's kernel has a little more LDS left than the BF16 one (whose
 // per-row head norms take some), so its tile cap and bitmap tier are larger.
 constexpr int CAP_I_DENSE = 6144;
-constexpr int CAP_J_I8 = 3424, CAP_J_BF16 = 3072;
+constexpr int CAP_J_I8 = 3808, CAP_J_BF16 = 3072;
 __host__ __device__ constexpr int cap_i_for(int dm) { return dm ? CAP_I_DENSE : CAP_I; }
 __host__ __device__ constexpr int cap_j_for(int dm) {
   return dm == DENSE_I8 ? CAP_J_I8 : dm == DENSE_BF16 ? CAP_J_BF16 : CAP_J;
