@@ -54,10 +54,12 @@ constexpr int DENSE_I8_FEATURES = 64;
 constexpr int DENSE_BF16_FEATURES = 128;
 // With a dense head the sparse blocks are smaller, and the LDS they free holds
 // the head operands: caps of the bitmap tier per band / tile / row.
-// The int8 head's kernel has a little more LDS left than the BF16 one (whose
-// per-row head norms take some), so its tile cap and bitmap tier are larger.
+// The tile cap (and with it the bitmap tier, since a row must fit one tile
+// block) takes the LDS the kernels have left (the BF16 kernel's per-row head
+// norms take some).  For the int8 head a larger tier measured slower: the
+// features beyond ~3800 cost more as lookups than as cursor products.
 constexpr int CAP_I_DENSE = 6144;
-constexpr int CAP_J_I8 = 3808, CAP_J_BF16 = 3072;
+constexpr int CAP_J_I8 = 3808, CAP_J_BF16 = 3680;
 __host__ __device__ constexpr int cap_i_for(int dm) { return dm ? CAP_I_DENSE : CAP_I; }
 __host__ __device__ constexpr int cap_j_for(int dm) {
   return dm == DENSE_I8 ? CAP_J_I8 : dm == DENSE_BF16 ? CAP_J_BF16 : CAP_J;

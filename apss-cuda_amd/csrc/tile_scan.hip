@@ -58,11 +58,13 @@ __device__ __forceinline__ float head_dot(const int32_t* __restrict__ ac,
 // the entries move as 32-bit / 128-bit words.  The loads for the next tile
 // are issued into registers before the current tile's product phase and
 // written to LDS after it, so their latency hides behind the products.
-// LDS output buffer (pairs): the int8-head kernel gives half of it to a
-// larger bitmap tier (its tile blocks take the LDS), the others keep 512
+// LDS staging buffers of emitted pairs and of error-band pairs.  The dense-head
+// kernels keep the first one small and give the LDS to a larger bitmap tier
+// (their tile blocks take it).
 template <int DM>
-constexpr int obuf_cap() { return DM == DENSE_I8 ? 256 : 512; }
-constexpr int BBUF_CAP = 256;               // LDS buffer of error-band pairs
+constexpr int obuf_cap() { return DM == DENSE_OFF ? 512 : 256; }
+template <int DM>
+constexpr int bbuf_cap() { return 256; }
 static_assert(KD_MAX < 65536 && CAP_J < 65536, "16-bit segment starts");
 
 // Per-mode sizes: bitmap-tier entries per thread of a band and of a tile.
@@ -273,11 +275,11 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
 // LDS staging of the workgroup's results: emitted pairs and error-band pairs
 // collect here (one LDS atomic per wave and round) and move to HBM in one
 // reserved range per buffer, so the global counters see one atomic per flush.
-template <int OC>
+template <int OC, int BC>
 struct StageBufs {
-  static constexpr int OUT_CAP = OC;
+  static constexpr int OUT_CAP = OC, BAND_CAP = BC;
   apss_pair out[OC];
-  unsigned long long band[BBUF_CAP];
+  unsigned long long band[BC];
   unsigned int n_out, n_band;          // may exceed the capacity: the excess went to HBM directly
   unsigned long long base_out, base_band;
 };
@@ -287,7 +289,7 @@ struct StageBufs {
 template <class SB>
 __device__ __forceinline__ void flush_stage(const ScanArgs& a, int tl, SB& sb) {
   const unsigned int no = sb.n_out < (unsigned)SB::OUT_CAP ? sb.n_out : (unsigned)SB::OUT_CAP;
-  const unsigned int nb = sb.n_band < BBUF_CAP ? sb.n_band : BBUF_CAP;
+  const unsigned int nb = sb.n_band < (unsigned)SB::BAND_CAP ? sb.n_band : (unsigned)SB::BAND_CAP;
   if (tl == 0) {
     sb.base_out = no ? atomicAdd(a.out_cnt, (unsigned long long)no) : 0ull;
     sb.base_band = nb ? atomicAdd(a.band_cnt, (unsigned long long)nb) : 0ull;
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ uint32_t bkt[CH_TILES + 1];             // cursor-product bucket starts of the chunk
   __shared__ uint32_t bfill[CH_TILES];               // bucket fill pointers while scattering
   __shared__ uint32_t wsum[SCAN_THREADS / 64 + 1];
-  __shared__ StageBufs<obuf_cap<DM>()> stage;        // results waiting for a flush
+  __shared__ StageBufs<obuf_cap<DM>(), bbuf_cap<DM>()> stage;  // results waiting for a flush
   // per-row values of tile J in two buffers (by J % 2): a wave that settles the
   // cells of tile J may still read them while the next tile is written
   __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[2][TILE_ROWS], hnJ[2][TILE_ROWS];
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // flush the staging buffers once one is half full: every append of the
     // earlier tile pairs is complete here (the next appends come after the
     // MFMA barrier, which also orders the reset of the counts before them)
-    if (stage.n_out >= obuf_cap<DM>() / 2 || stage.n_band >= BBUF_CAP / 2) flush_stage(a, tl, stage);
+    if (stage.n_out >= obuf_cap<DM>() / 2 || stage.n_band >= bbuf_cap<DM>() / 2) flush_stage(a, tl, stage);
 
     // this thread's first cursor product of tile J: its load completes behind
     // the bitmap-tier products.  It is issued before the prefetch of the next
@@ -788,7 +790,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       pr.score = (float)sd;
       stage_append(emit, pr, stage.out, obuf_cap<DM>(), &stage.n_out, a.out_cnt, a.out, a.out_cap);
       const unsigned long long key = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
-      stage_append(to_band, key, stage.band, BBUF_CAP, &stage.n_band, a.band_cnt, a.band, a.band_cap);
+      stage_append(to_band, key, stage.band, bbuf_cap<DM>(), &stage.n_band, a.band_cnt, a.band, a.band_cap);
     }
   }
   __syncthreads();
