@@ -133,12 +133,13 @@ struct TileSide {
   const float* dnorm = nullptr;   // norm of the head part of each row, rounded up (BF16)
 };
 
-// Cursor-tier inverted lists of the corpus side: {row, float bits of value},
-// ascending row inside each list.  ent and pos hold WALK_PAD entries past the
-// last list, so the band kernel's walks can read aligned 16-byte words.
+// Cursor-tier inverted lists of the corpus side, ascending row inside each
+// list, as two arrays per entry (value; J tile and row).  val and pos hold
+// WALK_PAD entries past the last list, so the band kernel's walks can read
+// aligned 16-byte words.
 struct CursorLists {
   const int64_t* ptr = nullptr;   // d+1
-  const int2* ent = nullptr;
+  const float* val = nullptr;     // value of each entry
   const uint32_t* pos = nullptr;  // J tile << 8 | row within the tile, per entry
 };
 

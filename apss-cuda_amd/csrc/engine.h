@@ -34,7 +34,7 @@ __global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, c
                              int32_t p, const int32_t* fmap, int32_t dm, int32_t B, uint32_t* dense,
                              float* dnorm);
 __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
-                             const int64_t* tstart, uint32_t* spos);
+                             const int64_t* tstart, uint32_t* spos, float* sval);
 template <int DM>
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_rescore(const unsigned long long* band, const unsigned long long* band_cnt,
@@ -114,6 +114,7 @@ struct Tiers {
   int64_t* sptr = nullptr;       // d+1
   int2* sent = nullptr;          // {row, value bits}
   uint32_t* spos = nullptr;      // J tile << 8 | row in tile, per entry (set per J tiling)
+  float* sval = nullptr;         // value of each entry (set with spos)
 };
 
 // One matrix side: CSR on the device and its norms.

@@ -553,10 +553,13 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
 
 void set_cursor_tiles(Arena& ar, hipStream_t st, Tiers& tr, const TileSet& tiles) {
   if (tr.spos) ar.release(tr.spos);
+  if (tr.sval) ar.release(tr.sval);
   tr.spos = ar.alloc<uint32_t>(tr.n_cursor_entries + WALK_PAD);
+  tr.sval = ar.alloc<float>(tr.n_cursor_entries + WALK_PAD);
   if (tr.n_cursor_entries > 0) {
     hipLaunchKernelGGL(k_cursor_pos, dim3(blocks_for(tr.n_cursor_entries, 256, 65536)), dim3(256),
-                       0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tiles.tstart, tr.spos);
+                       0, st, tr.n_cursor_entries, tr.sent, tiles.row2tile, tiles.tstart, tr.spos,
+                       tr.sval);
     APSS_HIP(hipGetLastError());
   }
 }
@@ -773,7 +776,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.I = make_view(I, bands);
   a.J = make_view(J, tiles);
   a.S.ptr = tr.sptr;
-  a.S.ent = tr.sent;
+  a.S.val = tr.sval;
   a.S.pos = tr.spos;
   a.KD = tr.KD;
   a.KW = tr.KW;

@@ -514,16 +514,19 @@ __global__ __launch_bounds__(256) void k_dense_rows(int64_t n, const int64_t* __
 }
 
 // J tile and row within that tile of every cursor-list entry, packed as
-// tile << 8 | local row (tiles have at most 64 rows, tile ids fit 24 bits).
+// tile << 8 | local row (tiles have at most 64 rows, tile ids fit 24 bits),
+// and the entry's value.
 __global__ __launch_bounds__(256) void k_cursor_pos(int64_t n, const int2* __restrict__ sent,
                                                     const int32_t* __restrict__ row2tile,
                                                     const int64_t* __restrict__ tstart,
-                                                    uint32_t* __restrict__ spos) {
+                                                    uint32_t* __restrict__ spos,
+                                                    float* __restrict__ sval) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t r = sent[e].x;
-    const int32_t t = row2tile[r];
-    spos[e] = ((uint32_t)t << 8) | (uint32_t)(r - tstart[t]);
+    const int2 x = sent[e];
+    const int32_t t = row2tile[x.x];
+    spos[e] = ((uint32_t)t << 8) | (uint32_t)(x.x - tstart[t]);
+    sval[e] = __int_as_float(x.y);  // the values alone, for the band kernel's walks
   }
 }
 

@@ -225,26 +225,22 @@ __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int
 // Writes the products and returns the first position not consumed; *t_next
 // receives the J tile of that position (INT32_MAX at the list end).
 __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
-                                            const int2* __restrict__ ent, int cur, int end,
+                                            const float* __restrict__ val, int cur, int end,
                                             int32_t cs, int32_t ce, int il, float x,
                                             uint32_t* bfill, int2* __restrict__ cprod,
                                             unsigned int& n_prod, int32_t* t_next) {
   const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
-  const int4* ent4 = reinterpret_cast<const int4*>(ent);  // two {row, value bits} per word
+  const float4* val4 = reinterpret_cast<const float4*>(val);
   int consumed = 0;
   for (int b = cur & ~3;; b += WALK_BATCH) {
     uint32_t pt[WALK_BATCH];
-    int32_t ev[WALK_BATCH];
+    float ev[WALK_BATCH];
 #pragma unroll
     for (int w = 0; w < WALK_BATCH / 4; ++w) {
       const uint4 x = pos4[(b >> 2) + w];
       pt[4 * w] = x.x; pt[4 * w + 1] = x.y; pt[4 * w + 2] = x.z; pt[4 * w + 3] = x.w;
-    }
-#pragma unroll
-    for (int w = 0; w < WALK_BATCH / 2; ++w) {
-      const int4 x = ent4[(b >> 1) + w];
-      ev[2 * w] = x.y;
-      ev[2 * w + 1] = x.w;
+      const float4 v = val4[(b >> 2) + w];
+      ev[4 * w] = v.x; ev[4 * w + 1] = v.y; ev[4 * w + 2] = v.z; ev[4 * w + 3] = v.w;
     }
     bool go = true;
     int used = 0;
@@ -257,7 +253,7 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
       go = go && t < ce;
       if (go) {
         const uint32_t slot = atomicAdd(&bfill[t - cs], 1u);
-        cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * __int_as_float(ev[q])));
+        cprod[slot] = make_int2(il + (int)(pt[q] & 0xffu) * TSTRIDE, (int)(x * ev[q]));
         ++used;
       } else if (was) {
         stop_t = t;
@@ -502,7 +498,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int4 r = a.I.srec[sb + k];
         int32_t tn;
         unsigned int np = 0;
-        const int cur = walk_scatter(a.S.pos, a.S.ent, st.x, st.y, ch_lo, ce, r.x,
+        const int cur = walk_scatter(a.S.pos, a.S.val, st.x, st.y, ch_lo, ce, r.x,
                                      __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn);
         scs[k] = make_int4(cur, st.y, tn, 0);
         if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
