@@ -257,7 +257,10 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long* counter, b
   int leader = __ffsll((long long)m) - 1;
   unsigned long long base = 0;
   if (lane_id() == leader) base = atomicAdd(counter, (unsigned long long)total);
-  base = __shfl(base, leader);
+  // leader is wave-uniform: read the two halves straight from its lane
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)base, leader);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(base >> 32), leader);
+  base = ((unsigned long long)hi << 32) | lo;
   return (long long)base + lane_rank(m);
 }
 

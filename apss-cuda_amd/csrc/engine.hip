@@ -736,20 +736,24 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     if (hi >= lo) c += (hi - lo + 1) * (j1 - 1) - (lo + hi) * (hi - lo + 1) / 2;
     return c;
   };
-  auto cut_items = [&](int32_t chunk, int32_t stride, std::vector<int4>& v) {
+  // items of every stride-th band; of each band's chunks, every `every`-th
+  // (starting at an offset that turns with the band); returns the tile pairs
+  auto cut_items = [&](int32_t chunk, int32_t stride, int32_t every, std::vector<int4>& v) {
     int64_t tp = 0;
-    for (int32_t b = 0; b < bands.T; b += stride)
-      for (int32_t j = jf[b]; j < je[b]; j += chunk) {
+    for (int32_t b = 0; b < bands.T; b += stride) {
+      int32_t k = (b / stride) % every;
+      for (int32_t j = jf[b] + k * chunk; j < je[b]; j += every * chunk) {
         v.push_back(make_int4(b, j, std::min(je[b], j + chunk), 0));
         tp += std::min(je[b], j + chunk) - j;
       }
+    }
     return tp;
   };
   const int64_t want_items = 8 * (int64_t)n_cu;
   const int32_t chunk = (int32_t)std::max<int64_t>(
       16, std::min<int64_t>(4096, (partner_tiles + want_items - 1) / want_items));
   std::vector<int4> hitems;
-  cut_items(chunk, 1, hitems);
+  cut_items(chunk, 1, 1, hitems);
   const int32_t n_items = (int32_t)hitems.size();
   int64_t host_cells = 0, host_dense = 0;
   for (const int4& it : hitems) {
@@ -838,11 +842,12 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     bcap = std::max<int64_t>(1 << 20, std::min<int64_t>(cap, 1 << 24));
   } else {
     // count-only launch over a sample of the bands (every stride-th band
-    // against all its partner tiles, in 64-tile chunks); its time, scaled by
-    // tile pairs, is also the autotuner's cost estimate of the configuration
+    // against its partner tiles in 64-tile chunks: every fourth chunk for the
+    // output estimate, all of them for the autotuner, which takes the time of
+    // the launch, scaled by tile pairs, as the cost of the configuration)
     const int32_t stride = std::max<int32_t>(1, bands.T / 128);
     std::vector<int4> sitems;
-    const int64_t stp = cut_items(64, stride, sitems);
+    const int64_t stp = cut_items(64, stride, rc.tune_only ? 1 : 4, sitems);
     int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
     APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
     a.items = sdev;
@@ -868,6 +873,9 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     }
     cap = (int64_t)std::min(cells, std::min(room, (double)h2[0] * scale * 1.25 + (double)(1 << 22)));
     bcap = (int64_t)((double)h2[1] * scale * 1.25) + (1 << 20);
+    if (trace_on())
+      std::fprintf(stderr, "[apss] estimate: %.4g pairs, %.4g band pairs (sample of %lld tile pairs)\n",
+                   (double)h2[0] * scale, (double)h2[1] * scale, (long long)stp);
     ph.mark("output estimate");
   }
   cap = std::max<int64_t>(cap, 1 << 16);
@@ -946,6 +954,9 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     APSS_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     const int64_t nout = (int64_t)h[0], nband = (int64_t)h[1];
+    if (trace_on())
+      std::fprintf(stderr, "[apss] scan: %lld pairs, %lld band pairs (room %lld, %lld)\n",
+                   (long long)nout, (long long)nband, (long long)cap, (long long)bcap);
     if (nout > cap || nband > bcap) {
       // capacity too small: every count is exact except the rescore's share of
       // a truncated band list; grow and run again

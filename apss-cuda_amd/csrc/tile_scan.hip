@@ -316,7 +316,7 @@ __device__ __forceinline__ void stage_append(bool want, const T& rec, T* buf, un
   const int leader = __ffsll((long long)m) - 1;
   unsigned int lb = 0;
   if (lane_id() == leader) lb = atomicAdd(n, (unsigned)tot);
-  lb = __shfl(lb, leader);
+  lb = (unsigned)__builtin_amdgcn_readlane((int)lb, leader);  // leader is wave-uniform
   const unsigned int pos = lb + (unsigned)lane_rank(m);
   if (want && pos < cap) buf[pos] = rec;
   if (lb + tot > cap) {
