@@ -336,8 +336,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
     throw Error{APSS_E_UNSUPPORTED, "cursor tier larger than 2^31 entries"};
   tr.fmap = ar.alloc<int32_t>(d);
   tr.sptr = ar.alloc<int64_t>(d + 1);
-  // padded: the band kernel's list walks read aligned groups of eight entries
-  tr.sent = ar.alloc<int2>(tr.n_cursor_entries + WALK_PAD);
+  tr.sent = ar.alloc<int2>(tr.n_cursor_entries);
   APSS_HIP(hipMemcpyAsync(tr.fmap, fm.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
   APSS_HIP(hipMemcpyAsync(tr.sptr, sp.data(), sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
   if (tr.n_cursor_entries > 0) {
