@@ -268,6 +268,19 @@ void sort_pairs(apss_pair* p, int64_t n);
 apss_pair* host_pairs_alloc(int64_t count);                       // room for max(count, 1) pairs
 apss_pair* host_pairs_grow(apss_pair* p, int64_t used, int64_t count);  // keeps the first `used`
 void host_pairs_free(apss_pair* p);
+
+// Host-to-device copies of pageable host arrays.  A large batch streams
+// through two pinned staging blocks: host threads fill one block while the
+// DMA engine reads the other (pageable arrays cannot be pinned in place:
+// hipHostRegister is refused on this platform).  Small batches, or no pinned
+// memory, go straight to hipMemcpyAsync.  The copies are complete on `st`
+// in stream order; the call returns once the staging blocks are free.
+struct H2DCopy {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st);
 inline int blocks_for(int64_t n, int threads, int cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;

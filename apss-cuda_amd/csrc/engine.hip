@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 
 #include "engine.h"
@@ -216,11 +217,13 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
     int64_t* rp = ar.alloc<int64_t>(s.n + 1);
     int32_t* col = ar.alloc<int32_t>(s.nnz);
     float* val = ar.alloc<float>(s.nnz);
-    if (s.n >= 0) APSS_HIP(hipMemcpyAsync(rp, X.row_ptr, sizeof(int64_t) * (s.n + 1), hipMemcpyHostToDevice, st));
+    std::vector<H2DCopy> cp;
+    if (s.n >= 0) cp.push_back({rp, X.row_ptr, sizeof(int64_t) * (size_t)(s.n + 1)});
     if (s.nnz > 0) {
-      APSS_HIP(hipMemcpyAsync(col, X.col, sizeof(int32_t) * s.nnz, hipMemcpyHostToDevice, st));
-      APSS_HIP(hipMemcpyAsync(val, X.val, sizeof(float) * s.nnz, hipMemcpyHostToDevice, st));
+      cp.push_back({col, X.col, sizeof(int32_t) * (size_t)s.nnz});
+      cp.push_back({val, X.val, sizeof(float) * (size_t)s.nnz});
     }
+    h2d_copies(cp, st);
     s.rp = rp;
     s.col = col;
     s.val = val;
@@ -1047,6 +1050,95 @@ apss_pair* host_pairs_alloc(int64_t count) {
   return q;
 }
 
+namespace {
+bool host_pairs_pinned(const void* p) {
+  PinnedPool& pp = pinned_pool();
+  std::lock_guard<std::mutex> lk(pp.mu);
+  return pp.live.count(const_cast<void*>(p)) != 0;
+}
+
+// memcpy of `pieces` (block offset, source, bytes) into `blk`, the bytes split
+// evenly over up to eight host threads
+struct Piece {
+  size_t at;
+  const char* src;
+  size_t bytes;
+};
+void fill_block(char* blk, const std::vector<Piece>& pieces, size_t total) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = total < ((size_t)4 << 20) ? 1 : (int)std::min(8u, hw);
+  auto work = [&](int t) {
+    const size_t lo = total * (size_t)t / (size_t)nt, hi = total * (size_t)(t + 1) / (size_t)nt;
+    for (const Piece& q : pieces) {
+      const size_t a = std::max(lo, q.at), b = std::min(hi, q.at + q.bytes);
+      if (a < b) std::memcpy(blk + a, q.src + (a - q.at), b - a);
+    }
+  };
+  std::vector<std::thread> th;
+  int t = 1;
+  try {
+    for (; t < nt; ++t) th.emplace_back(work, t);
+  } catch (...) {  // no thread to spare: this thread copies the rest
+    for (; t < nt; ++t) work(t);
+  }
+  work(0);
+  for (auto& x : th) x.join();
+}
+}  // namespace
+
+void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st) {
+  constexpr size_t STAGED_MIN = (size_t)16 << 20;  // smaller batches: plain copies
+  constexpr size_t CHUNK = (size_t)32 << 20;       // bytes per staging block
+  size_t total = 0;
+  for (const H2DCopy& c : list) total += c.bytes;
+  auto direct = [&]() {
+    for (const H2DCopy& c : list)
+      if (c.bytes) APSS_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st));
+  };
+  if (total < STAGED_MIN) return direct();
+  const int64_t slots = (int64_t)(CHUNK / sizeof(apss_pair)) + 1;
+  struct Stage {
+    apss_pair* blk[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Stage() {
+      for (int b = 0; b < 2; ++b) {
+        if (ev[b]) {
+          (void)hipEventSynchronize(ev[b]);  // the DMA of the block is done
+          (void)hipEventDestroy(ev[b]);
+        }
+        host_pairs_free(blk[b]);
+      }
+    }
+  } sg;
+  for (int b = 0; b < 2; ++b) sg.blk[b] = host_pairs_alloc(slots);
+  if (!host_pairs_pinned(sg.blk[0]) || !host_pairs_pinned(sg.blk[1])) return direct();
+  for (int b = 0; b < 2; ++b) APSS_HIP(hipEventCreateWithFlags(&sg.ev[b], hipEventDisableTiming));
+  // the list as one byte stream, cut in CHUNK-sized blocks
+  size_t li = 0, lo = 0;  // current copy and offset inside it
+  for (int k = 0; li < list.size(); ++k) {
+    const int b = k & 1;
+    if (k >= 2) APSS_HIP(hipEventSynchronize(sg.ev[b]));  // block b's last DMA is done
+    std::vector<Piece> pieces;
+    std::vector<H2DCopy> dmas;
+    size_t fill = 0;
+    while (li < list.size() && fill < CHUNK) {
+      const H2DCopy& c = list[li];
+      const size_t n = std::min(c.bytes - lo, CHUNK - fill);
+      if (n) {
+        pieces.push_back({fill, (const char*)c.src + lo, n});
+        dmas.push_back({(char*)c.dst + lo, (const char*)sg.blk[b] + fill, n});
+      }
+      fill += n;
+      lo += n;
+      if (lo == c.bytes) { ++li; lo = 0; }
+    }
+    fill_block((char*)sg.blk[b], pieces, fill);
+    for (const H2DCopy& d : dmas)
+      APSS_HIP(hipMemcpyAsync(d.dst, d.src, d.bytes, hipMemcpyHostToDevice, st));
+    APSS_HIP(hipEventRecord(sg.ev[b], st));
+  }
+}
+
 apss_pair* host_pairs_grow(apss_pair* p, int64_t used, int64_t count) {
   if (!p) return host_pairs_alloc(count);
   size_t have = 0;
@@ -1265,16 +1357,18 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   int64_t* drp = ar.alloc<int64_t>(n_all + 1);
   int32_t* dcol = ar.alloc<int32_t>(nnz_all);
   float* dval = ar.alloc<float>(nnz_all);
-  APSS_HIP(hipMemcpyAsync(drp, rp.data(), sizeof(int64_t) * (n_all + 1), hipMemcpyHostToDevice, st));
+  std::vector<H2DCopy> cp;
+  cp.push_back({drp, rp.data(), sizeof(int64_t) * (size_t)(n_all + 1)});
   e0 = 0;
   for (int32_t k = 0; k < count; ++k) {
     const apss_csr& A = *X[k];
     if (A.nnz) {
-      APSS_HIP(hipMemcpyAsync(dcol + e0, A.col, sizeof(int32_t) * A.nnz, hipMemcpyHostToDevice, st));
-      APSS_HIP(hipMemcpyAsync(dval + e0, A.val, sizeof(float) * A.nnz, hipMemcpyHostToDevice, st));
+      cp.push_back({dcol + e0, A.col, sizeof(int32_t) * (size_t)A.nnz});
+      cp.push_back({dval + e0, A.val, sizeof(float) * (size_t)A.nnz});
     }
     e0 += A.nnz;
   }
+  h2d_copies(cp, st);
   apss_csr all{n_all, d, nnz_all, drp, dcol, dval, 1, 0};
   Side s = prepare_side(ar, st, all, p);
   const double ms_upload = tm.stop_ms();
