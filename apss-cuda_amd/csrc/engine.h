@@ -281,6 +281,10 @@ struct H2DCopy {
   size_t bytes;
 };
 void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st);
+// Device-to-host copy into `dst`, complete on return.  A large copy into
+// pageable memory runs through two pinned staging blocks: the DMA engine
+// fills one while host threads empty the other into `dst`.
+void d2h_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 inline int blocks_for(int64_t n, int threads, int cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;

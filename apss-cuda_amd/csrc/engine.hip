@@ -230,34 +230,23 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
   }
   s.norm = ar.alloc<float>(s.n);
   s.hnorm = ar.alloc<float>(s.n);
-  uint32_t* err = ar.alloc<uint32_t>(1);
-  fill_i32(st, (int32_t*)err, 1, 0);
+  uint32_t* err = ar.alloc<uint32_t>(3);  // error bits, longest row, largest norm
+  fill_i32(st, (int32_t*)err, 3, 0);
   if (s.n > 0) {
-    hipLaunchKernelGGL(k_validate_norms, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n,
+    // a grid of at most 2048 blocks: each wave takes many rows, so few waves
+    // post their maxima with atomics
+    hipLaunchKernelGGL(k_validate_norms, dim3(blocks_for(s.n, 4, 2048)), dim3(256), 0, st, s.n,
                        s.d, s.nnz, s.rp, s.col, s.val, p, s.norm, s.hnorm, err);
     APSS_HIP(hipGetLastError());
   }
-  uint32_t herr = 0;
-  APSS_HIP(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
+  uint32_t herr[3] = {0, 0, 0};
+  APSS_HIP(hipMemcpyAsync(herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
   APSS_HIP(hipStreamSynchronize(st));
   ar.release(err);
-  if (herr) throw Error{APSS_E_INVALID, "invalid CSR:" + verr_text(herr)};
+  if (herr[0]) throw Error{APSS_E_INVALID, "invalid CSR:" + verr_text(herr[0])};
   // maxima for the global fast-reject bound
-  if (s.n > 0) {
-    std::vector<int64_t> hrp((size_t)s.n + 1);
-    std::vector<float> hn((size_t)s.n);
-    APSS_HIP(hipMemcpyAsync(hrp.data(), s.rp, sizeof(int64_t) * (s.n + 1), hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipMemcpyAsync(hn.data(), s.norm, sizeof(float) * s.n, hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipStreamSynchronize(st));
-    int64_t ml = 0;
-    float mn = 0.f;
-    for (int64_t i = 0; i < s.n; ++i) {
-      ml = std::max(ml, hrp[i + 1] - hrp[i]);
-      mn = std::max(mn, hn[i]);
-    }
-    s.maxlen = (int32_t)ml;
-    s.maxnorm = mn;
-  }
+  s.maxlen = (int32_t)herr[1];
+  std::memcpy(&s.maxnorm, &herr[2], sizeof(float));
   return s;
 }
 
@@ -1064,31 +1053,64 @@ struct Piece {
   const char* src;
   size_t bytes;
 };
-void fill_block(char* blk, const std::vector<Piece>& pieces, size_t total) {
+template <class Work>
+void on_threads(size_t total, Work work) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const int nt = total < ((size_t)4 << 20) ? 1 : (int)std::min(8u, hw);
-  auto work = [&](int t) {
-    const size_t lo = total * (size_t)t / (size_t)nt, hi = total * (size_t)(t + 1) / (size_t)nt;
-    for (const Piece& q : pieces) {
-      const size_t a = std::max(lo, q.at), b = std::min(hi, q.at + q.bytes);
-      if (a < b) std::memcpy(blk + a, q.src + (a - q.at), b - a);
-    }
+  auto part = [&](int t) {
+    work(total * (size_t)t / (size_t)nt, total * (size_t)(t + 1) / (size_t)nt);
   };
   std::vector<std::thread> th;
   int t = 1;
   try {
-    for (; t < nt; ++t) th.emplace_back(work, t);
+    for (; t < nt; ++t) th.emplace_back(part, t);
   } catch (...) {  // no thread to spare: this thread copies the rest
-    for (; t < nt; ++t) work(t);
+    for (; t < nt; ++t) part(t);
   }
-  work(0);
+  part(0);
   for (auto& x : th) x.join();
 }
+void fill_block(char* blk, const std::vector<Piece>& pieces, size_t total) {
+  on_threads(total, [&](size_t lo, size_t hi) {
+    for (const Piece& q : pieces) {
+      const size_t a = std::max(lo, q.at), b = std::min(hi, q.at + q.bytes);
+      if (a < b) std::memcpy(blk + a, q.src + (a - q.at), b - a);
+    }
+  });
+}
+
+constexpr size_t STAGED_MIN = (size_t)16 << 20;  // smaller copies go direct
+constexpr size_t CHUNK = (size_t)32 << 20;       // bytes per staging block
+
+// two pinned staging blocks and their DMA events
+struct Stage {
+  apss_pair* blk[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ok = false;  // both blocks pinned
+  Stage() {
+    const int64_t slots = (int64_t)(CHUNK / sizeof(apss_pair)) + 1;
+    for (int b = 0; b < 2; ++b) blk[b] = host_pairs_alloc(slots);
+    ok = host_pairs_pinned(blk[0]) && host_pairs_pinned(blk[1]);
+    for (int b = 0; b < 2 && ok; ++b)
+      if (hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        ev[b] = nullptr;
+        ok = false;  // the copies go direct
+      }
+  }
+  ~Stage() {
+    for (int b = 0; b < 2; ++b) {
+      if (ev[b]) {
+        (void)hipEventSynchronize(ev[b]);  // no DMA still uses the block
+        (void)hipEventDestroy(ev[b]);
+      }
+      host_pairs_free(blk[b]);
+    }
+  }
+};
 }  // namespace
 
 void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st) {
-  constexpr size_t STAGED_MIN = (size_t)16 << 20;  // smaller batches: plain copies
-  constexpr size_t CHUNK = (size_t)32 << 20;       // bytes per staging block
   size_t total = 0;
   for (const H2DCopy& c : list) total += c.bytes;
   auto direct = [&]() {
@@ -1096,23 +1118,8 @@ void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st) {
       if (c.bytes) APSS_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st));
   };
   if (total < STAGED_MIN) return direct();
-  const int64_t slots = (int64_t)(CHUNK / sizeof(apss_pair)) + 1;
-  struct Stage {
-    apss_pair* blk[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    ~Stage() {
-      for (int b = 0; b < 2; ++b) {
-        if (ev[b]) {
-          (void)hipEventSynchronize(ev[b]);  // the DMA of the block is done
-          (void)hipEventDestroy(ev[b]);
-        }
-        host_pairs_free(blk[b]);
-      }
-    }
-  } sg;
-  for (int b = 0; b < 2; ++b) sg.blk[b] = host_pairs_alloc(slots);
-  if (!host_pairs_pinned(sg.blk[0]) || !host_pairs_pinned(sg.blk[1])) return direct();
-  for (int b = 0; b < 2; ++b) APSS_HIP(hipEventCreateWithFlags(&sg.ev[b], hipEventDisableTiming));
+  Stage sg;
+  if (!sg.ok) return direct();
   // the list as one byte stream, cut in CHUNK-sized blocks
   size_t li = 0, lo = 0;  // current copy and offset inside it
   for (int k = 0; li < list.size(); ++k) {
@@ -1136,6 +1143,36 @@ void h2d_copies(const std::vector<H2DCopy>& list, hipStream_t st) {
     for (const H2DCopy& d : dmas)
       APSS_HIP(hipMemcpyAsync(d.dst, d.src, d.bytes, hipMemcpyHostToDevice, st));
     APSS_HIP(hipEventRecord(sg.ev[b], st));
+  }
+}
+
+void d2h_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes < STAGED_MIN || host_pairs_pinned(dst)) {
+    if (bytes) APSS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  Stage sg;
+  if (!sg.ok) {
+    APSS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  const size_t nch = (bytes + CHUNK - 1) / CHUNK;
+  auto issue = [&](size_t k) {  // chunk k into block k % 2
+    const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off);
+    APSS_HIP(hipMemcpyAsync(sg.blk[k & 1], (const char*)src + off, len, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipEventRecord(sg.ev[k & 1], st));
+  };
+  issue(0);
+  for (size_t k = 0; k < nch; ++k) {
+    // block (k+1) % 2 was emptied in the previous step
+    if (k + 1 < nch) issue(k + 1);
+    APSS_HIP(hipEventSynchronize(sg.ev[k & 1]));
+    const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off);
+    const char* from = (const char*)sg.blk[k & 1];
+    char* to = (char*)dst + off;
+    on_threads(len, [&](size_t lo, size_t hi) { std::memcpy(to + lo, from + lo, hi - lo); });
   }
 }
 
@@ -1255,9 +1292,12 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
     out->on_device = 1;
   } else {
     apss_pair* h = host_pairs_alloc(so.count);
-    if (so.count)
-      APSS_HIP(hipMemcpyAsync(h, so.pairs, sizeof(apss_pair) * so.count, hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipStreamSynchronize(st));
+    try {
+      d2h_copy(h, so.pairs, sizeof(apss_pair) * (size_t)so.count, st);
+    } catch (...) {
+      host_pairs_free(h);
+      throw;
+    }
     if (o->sort_output) sort_pairs(h, so.count);
     if (o->output_on_device) {
       apss_pair* d = so.pairs;
@@ -1298,9 +1338,12 @@ StreamScope::~StreamScope() {
 void finish_result(hipStream_t st, const apss_pair* dpairs, int64_t count, bool sort,
                    apss_result* out) {
   apss_pair* h = host_pairs_alloc(count);
-  if (count)
-    APSS_HIP(hipMemcpyAsync(h, dpairs, sizeof(apss_pair) * count, hipMemcpyDeviceToHost, st));
-  APSS_HIP(hipStreamSynchronize(st));
+  try {
+    d2h_copy(h, dpairs, sizeof(apss_pair) * (size_t)count, st);
+  } catch (...) {
+    host_pairs_free(h);
+    throw;
+  }
   if (sort) sort_pairs(h, count);
   out->pairs = h;
   out->count = count;

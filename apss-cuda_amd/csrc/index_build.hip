@@ -44,7 +44,8 @@ __global__ void k_copy_i64(int64_t* dst, const int64_t* src, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// validation + norms: one wave per row
+// validation + norms: one wave per row.  err[0] collects the error bits,
+// err[1] and err[2] (zeroed by the caller) the longest row and the largest norm.
 
 __device__ __forceinline__ float round_up_f(double s) {
   float f = (float)s;
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void k_validate_norms(int64_t n, int64_t d, in
                                                         uint32_t* __restrict__ err) {
   const int lane = lane_id();
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  uint32_t max_len = 0, max_norm = 0;  // of this wave's rows (norm as float bits, >= 0)
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
     int64_t a = rp[r], b = rp[r + 1];
     uint32_t e_bits = 0;
@@ -90,10 +92,19 @@ __global__ __launch_bounds__(256) void k_validate_norms(int64_t n, int64_t d, in
     if (lane == 0) {
       // rows must keep |score| <= ||x|| ||y|| far from FP32 overflow
       if (!(ss < 1.3e36)) e_bits |= VERR_NORM;
-      norm[r] = round_up_f(sqrt(ss));
+      const float nr = round_up_f(sqrt(ss));
+      norm[r] = nr;
       hnorm[r] = round_up_f(sqrt(hs));
       if (e_bits) atomicOr(err, e_bits);
+      const int64_t len = b - a;
+      max_len = max(max_len, (uint32_t)(len < 0x7fffffff ? len : 0x7fffffff));
+      max_norm = max(max_norm, __float_as_uint(nr));
     }
+  }
+  // the maxima over all rows: err[1] the longest row, err[2] the largest norm
+  if (lane == 0 && (max_len || max_norm)) {
+    atomicMax(err + 1, max_len);
+    atomicMax(err + 2, max_norm);
   }
 }
 
