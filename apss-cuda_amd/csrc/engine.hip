@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <sys/mman.h>
 #include <thread>
 #include <unordered_map>
 
@@ -1034,7 +1035,16 @@ apss_pair* host_pairs_alloc(int64_t count) {
     }
     (void)hipGetLastError();  // no pinned memory: fall back to pageable
   }
-  apss_pair* q = (apss_pair*)std::malloc(bytes);
+  apss_pair* q = nullptr;
+  if (bytes > PINNED_MAX) {
+    // 2 MiB-aligned and backed by huge pages where the kernel allows it: the
+    // first touch of a multi-GB result then faults once per 2 MiB, not per 4 KiB
+    const size_t huge = (size_t)2 << 20;
+    q = (apss_pair*)std::aligned_alloc(huge, (bytes + huge - 1) / huge * huge);
+    if (q) (void)madvise(q, (bytes + huge - 1) / huge * huge, MADV_HUGEPAGE);
+  } else {
+    q = (apss_pair*)std::malloc(bytes);
+  }
   if (!q) throw Error{APSS_E_NOMEM, "host allocation of the result failed"};
   return q;
 }
