@@ -116,3 +116,36 @@ def test_large_t09_pair_set(large):
     count, _, keys, scores, info = stream(large, 0.9)
     assert check_sample(large, 0.9, keys, scores, rel=1e-6) >= 0
     assert count > 0
+
+
+def digest(pairs):
+    """Order-independent digest of a pair array: count and the sum mod 2^64
+    of a hash of (i, j, score bits)."""
+    i = pairs["i"].astype(np.uint64)
+    j = pairs["j"].astype(np.uint64)
+    s = pairs["score"].view(np.uint32).astype(np.uint64)
+    h = (i * K1) ^ (j * K2) ^ (s * np.uint64(0x165667B19E3779F9))
+    return len(pairs), int(h.sum(dtype=np.uint64))
+
+
+def test_host_result_above_2gib_matches_device_result(large):
+    # A host result above 2 GiB comes back through pinned staging blocks into
+    # a huge-page-backed array (the smaller ones land in pinned blocks
+    # directly); it must hold the same pairs and scores as the result left in
+    # device memory.  The matrix goes in from host memory, through the staged
+    # upload.
+    rp, col, val, d, _ = large
+    t = 0.7
+    host_pairs, info = apss.run(rp, col, val, d, t, 0, mfma_head=0)
+    print(f"host result: {info['count']} pairs, download {info['ms_download']:.0f} ms", flush=True)
+    assert info["count"] * apss.PAIR_DTYPE.itemsize > (2 << 30)
+    hip = apss.Hip()
+    X = apss.make_csr(rp, col, val, d)
+    r = apss.run_csr(X, t, 0, output_on_device=1, mfma_head=0)
+    try:
+        assert r.on_device == 1
+        dev = np.zeros(r.count, apss.PAIR_DTYPE)
+        hip.d2h(dev, r.pairs)
+    finally:
+        apss.lib().apss_result_free(apss.ctypes.byref(r))
+    assert digest(host_pairs) == digest(dev)
