@@ -2,6 +2,7 @@
 #pragma once
 #include <functional>
 #include <map>
+#include <mutex>
 #include <memory>
 
 #include "apss_internal.h"
@@ -46,24 +47,34 @@ __global__ void k_rescore(const unsigned long long* band, const unsigned long lo
 
 // ---- device memory --------------------------------------------------------
 
-// Device blocks kept for reuse by one serving slot: blocks freed by an arena
-// that draws on the pool go back to it instead of to hipFree, so the request
-// loop stops calling hipMalloc / hipFree (which synchronise the device) once
-// the pool has grown to its working set.  Not thread-safe: one pool per slot.
+// Device blocks kept for reuse: blocks freed by an arena that draws on the
+// pool go back to it instead of to hipFree, so repeated calls stop calling
+// hipMalloc / hipFree (which synchronise the device) once the pool has grown
+// to their working set.  Each serving slot has a pool of its own; apss_run and
+// apss_run_batched share one per device (run_pool), which keeps at most
+// RUN_POOL_KEEP idle bytes.  Thread-safe.
 class DevicePool {
  public:
-  DevicePool() = default;
+  explicit DevicePool(size_t keep_max = 0) : keep_max_(keep_max) {}  // 0: no limit
   ~DevicePool();
   DevicePool(const DevicePool&) = delete;
   DevicePool& operator=(const DevicePool&) = delete;
   // A free block of at least `bytes` (and at most twice that), or nullptr.
   void* take(size_t bytes, size_t* got);
-  void give(void* p, size_t bytes) { free_.emplace(bytes, p); }
+  // Keeps the block, or frees it when the pool would exceed its limit.
+  void give(void* p, size_t bytes);
+  // Frees every idle block (before retrying a failed allocation).
+  void trim();
   int64_t held() const;
 
  private:
+  mutable std::mutex mu_;
   std::multimap<size_t, void*> free_;
+  size_t idle_ = 0;
+  size_t keep_max_;
 };
+constexpr size_t RUN_POOL_KEEP = (size_t)8 << 30;
+DevicePool& run_pool(int device);
 
 // Owns every device allocation of one call (or one index); enforces a byte
 // budget and frees everything on destruction (or returns it to a pool).

@@ -32,18 +32,53 @@ DevicePool::~DevicePool() {
 }
 
 void* DevicePool::take(size_t bytes, size_t* got) {
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = free_.lower_bound(bytes);
   if (it == free_.end() || it->first > 2 * bytes + (1 << 20)) return nullptr;
   void* p = it->second;
   *got = it->first;
+  idle_ -= it->first;
   free_.erase(it);
   return p;
 }
 
+void DevicePool::give(void* p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (keep_max_ == 0 || idle_ + bytes <= keep_max_) {
+      free_.emplace(bytes, p);
+      idle_ += bytes;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void DevicePool::trim() {
+  std::multimap<size_t, void*> f;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    f.swap(free_);
+    idle_ = 0;
+  }
+  for (auto& b : f) (void)hipFree(b.second);
+}
+
 int64_t DevicePool::held() const {
-  int64_t s = 0;
-  for (auto& b : free_) s += (int64_t)b.first;
-  return s;
+  std::lock_guard<std::mutex> lk(mu_);
+  return (int64_t)idle_;
+}
+
+DevicePool& run_pool(int device) {
+  // created on first use and never destroyed: the blocks stay with the
+  // process (its exit releases them), and no static destructor runs after
+  // the HIP runtime has shut down
+  static std::mutex mu;
+  static std::map<int, DevicePool*>* pools = new std::map<int, DevicePool*>;
+  std::lock_guard<std::mutex> lk(mu);
+  DevicePool*& p = (*pools)[device];
+  if (!p) p = new DevicePool(RUN_POOL_KEEP);
+  return *p;
 }
 
 Arena::~Arena() {
@@ -65,7 +100,14 @@ void* Arena::raw(size_t bytes) {
   if (pool_) p = pool_->take(bytes, &got);
   if (!p) {
     got = bytes;
-    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+    bool ok = hipMalloc(&p, bytes) == hipSuccess && p;
+    if (!ok && pool_ && pool_->held() > 0) {  // idle pooled blocks may be in the way
+      (void)hipGetLastError();
+      pool_->trim();
+      p = nullptr;
+      ok = hipMalloc(&p, bytes) == hipSuccess && p;
+    }
+    if (!ok) {
       (void)hipGetLastError();
       throw Error{APSS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
     }
@@ -1245,7 +1287,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
     ~StreamGuard() { if (own) (void)hipStreamDestroy(s); }
   } sg{st, own_stream};
   auto t0 = std::chrono::steady_clock::now();
-  Arena ar(o->device, o->mem_budget);
+  Arena ar(o->device, o->mem_budget, &run_pool(o->device));
   // p < 0: autotune.  The split is p = 0 (every feature on the tiered
   // inverted-list / dense-head path; p > 0 adds the reference's forward-list
   // head dot, which loses on Zipf data); the tuner picks the tiering.
@@ -1404,7 +1446,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   off[count] = r0;
   StreamScope ss(o->stream, o->device);
   hipStream_t st = ss.s;
-  Arena ar(o->device, o->mem_budget);
+  Arena ar(o->device, o->mem_budget, &run_pool(o->device));
   Timer tm(st);
   tm.start();
   int64_t* drp = ar.alloc<int64_t>(n_all + 1);

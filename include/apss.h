@@ -86,7 +86,10 @@ typedef struct {
   int32_t rescore_all;     /* 1: every emitted score is the FP64 rescore */
   int32_t output_on_device;/* 1: result.pairs is device memory (free with apss_result_free) */
   int32_t dense_features;  /* cap of the bitmap tier (0 = automatic) */
-  int64_t mem_budget;      /* device bytes a call may allocate (0 = no limit) */
+  int64_t mem_budget;      /* device bytes a call may allocate (0 = no limit).  Device
+                              blocks a call frees are kept for later calls on the same
+                              device, up to 8 GiB per device, for the life of the process
+                              (a failed allocation releases them first). */
   void* stream;            /* hipStream_t to run on (NULL = library-owned stream) */
   int32_t mfma_head;       /* dense MFMA head of the most frequent tail features:
                               0 = on (64 features as exact int8 limbs in FP32 mode,
