@@ -123,7 +123,7 @@ struct TileSide {
   const uint16_t* dfeat = nullptr;  // bitmap index of each bitmap-tier entry
   // cursor-tier entries of this side (one record per entry, grouped by tile)
   const int64_t* sbase = nullptr; // T+1
-  const int4* srec = nullptr;     // {local row, float bits of x, first cursor, end}
+  const int4* srec = nullptr;     // {local row, float bits of x, list begin, list end}
   // bitmap tier, per tile: start of every feature's segment in the tile
   // block, indexed by bitmap index (KD + 1 entries, row stride kds)
   const uint16_t* dseg = nullptr;

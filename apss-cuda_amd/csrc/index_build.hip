@@ -347,15 +347,6 @@ __global__ __launch_bounds__(1024) void k_tile_ptr_scan(BuildSide s) {
   if (threadIdx.x == 0) s.ctot[tile] = present + 1;  // starts + end sentinel
 }
 
-__device__ __forceinline__ int64_t lower_bound_row(const int2* a, int64_t lo, int64_t hi,
-                                                   int32_t key) {
-  while (lo < hi) {
-    int64_t m = (lo + hi) >> 1;
-    if (a[m].x < key) lo = m + 1; else hi = m;
-  }
-  return lo;
-}
-
 // Scatter bitmap-tier entries into their tile blocks and write the cursor
 // records of this side's cursor-tier entries.
 __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
@@ -388,10 +379,11 @@ __global__ __launch_bounds__(256) void k_tile_scatter(BuildSide s) {
       s.dval[dbase + q] = s.val[e];
       s.dfeat[dbase + q] = (uint16_t)f;
     } else if (s.srec) {  // cursor records only for the band (I) side
+      // the record spans its whole list: the band kernel positions it at each
+      // work item's first tile, and in a self-join the cells j <= i that the
+      // diagonal tiles receive are never emitted
       unsigned int q = atomicAdd(&sfill, 1u);
-      int64_t a = s.sptr[c], b = s.sptr[c + 1];
-      int64_t start = s.self ? lower_bound_row(s.sent, a, b, (int32_t)(r0 + lo + 1)) : a;
-      s.srec[sbase + q] = make_int4(lo, __float_as_int(s.val[e]), (int)start, (int)b);
+      s.srec[sbase + q] = make_int4(lo, __float_as_int(s.val[e]), (int)s.sptr[c], (int)s.sptr[c + 1]);
     }
   }
 }
