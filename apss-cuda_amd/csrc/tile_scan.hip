@@ -145,6 +145,17 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, const TileMeta&
   pf.rp1 = reinterpret_cast<const uint32_t*>(S.rp + r + 1)[0];
 }
 
+// The J side of the kernel's arguments, copied from the kernarg segment where
+// it is used: the opaque pointer keeps the compiler from hoisting the loads
+// out of the tile loop, so these pointers hold no scalar registers between
+// their uses (fewer scalar registers spill to vector-register lanes).
+typedef __attribute__((address_space(4))) const ScanArgs* KArgs;
+__device__ __forceinline__ TileSide kargs_J() {
+  KArgs p = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return ((const ScanArgs*)(const void*)p)->J;
+}
+
 // Workgroup barrier for hand-offs through LDS only: it waits for this wave's
 // LDS operations but not for its global loads and stores in flight (the
 // prefetch of the next tile, output stores), which __syncthreads() would.
@@ -511,7 +522,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const int nJ = pf.m.nrows;
 
     // ---- write the prefetched tile J into LDS --------------------------------
-    if (4 * tl < a.J.kds) reinterpret_cast<uint2*>(cJ)[tl] = pf.seg;
+    const int32_t kds = DM != DENSE_BF16 ? kargs_J().kds : a.J.kds;
+    if (4 * tl < kds) reinterpret_cast<uint2*>(cJ)[tl] = pf.seg;
 #pragma unroll
     for (int i = 0; i < Z::QPT; ++i) {
       const int k = tl + i * SCAN_THREADS;
@@ -558,8 +570,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // the loads it needs.
     {
       const int32_t t1 = J + 1 < J_end ? J + 1 : J;
-      prefetch_tile<DM>(a.J, meta_next, t1, tl, pf);
-      meta_next = tile_meta(a.J, J + 2 < J_end ? J + 2 : J_end - 1);
+      if constexpr (DM != DENSE_BF16) {
+        // (1.3% faster for the int8 kernel; the BF16 kernel's copy would
+        // live in scratch memory, so it reads the hoisted argument)
+        const TileSide Js = kargs_J();
+        prefetch_tile<DM>(Js, meta_next, t1, tl, pf);
+        meta_next = tile_meta(Js, J + 2 < J_end ? J + 2 : J_end - 1);
+      } else {
+        prefetch_tile<DM>(a.J, meta_next, t1, tl, pf);
+        meta_next = tile_meta(a.J, J + 2 < J_end ? J + 2 : J_end - 1);
+      }
     }
 
     // ---- 2. bitmap tier: every band entry meets its feature's J segment -----
