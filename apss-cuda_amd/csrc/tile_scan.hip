@@ -122,27 +122,33 @@ template <int DM>
 __device__ __forceinline__ void prefetch_tile(const TileSide& S, const TileMeta& m, int32_t t,
                                               int tid, Prefetch<DM>& pf) {
   pf.m = m;
-  pf.seg = reinterpret_cast<const uint2*>(S.dseg + (int64_t)t * S.kds)[4 * tid < S.kds ? tid : 0];
-  const int64_t q0 = m.d0 >> 2;
+  // uniform tile bases (scalar registers) and 32-bit lane offsets
+  const uint2* segb = reinterpret_cast<const uint2*>(S.dseg + (int64_t)t * S.kds);
+  pf.seg = segb[4 * tid < S.kds ? (unsigned)tid : 0u];
+  const uint32_t* rowb = reinterpret_cast<const uint32_t*>(S.drow) + (m.d0 >> 2);
+  const float4* valb = reinterpret_cast<const float4*>(S.dval) + (m.d0 >> 2);
 #pragma unroll
   for (int i = 0; i < Sizes<DM>::QPT; ++i) {
     const int k = tid + i * SCAN_THREADS;
-    const int64_t q = k < m.nd4 ? q0 + k : 0;
-    pf.rows[i] = reinterpret_cast<const uint32_t*>(S.drow)[q];
-    pf.vals[i] = reinterpret_cast<const float4*>(S.dval)[q];
+    const unsigned q = k < m.nd4 ? (unsigned)k : 0u;
+    pf.rows[i] = rowb[q];
+    pf.vals[i] = valb[q];
   }
   if constexpr (DM != DENSE_OFF) {  // thread -> (row tid / 16, chunk tid % 16)
     const int row = tid >> 4;
-    pf.dense = S.dense[(m.r0 + (row < m.nrows ? row : 0)) * 16 + (tid & 15)];
+    const uint4* db = S.dense + m.r0 * 16;
+    pf.dense = db[(row < m.nrows ? (unsigned)row : 0u) * 16u + (unsigned)(tid & 15)];
   }
-  // per-row values, one per thread of the first four waves (the pointer is
-  // uniform per wave)
-  const int row = tid & 63, which = tid >> 6;
-  const int64_t r = m.r0 + (row < m.nrows ? row : 0);
+  // per-row values, one per thread of the first four waves (the array is
+  // chosen per wave)
+  const int row = tid & 63;
+  const int which = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const unsigned r = row < m.nrows ? (unsigned)row : 0u;
   const float* src = which == 1 ? S.hnorm : (which == 3 && DM == DENSE_BF16) ? S.dnorm : S.norm;
-  pf.aux = __float_as_uint(src[r]);
-  pf.rp0 = reinterpret_cast<const uint32_t*>(S.rp + r)[0];
-  pf.rp1 = reinterpret_cast<const uint32_t*>(S.rp + r + 1)[0];
+  pf.aux = __float_as_uint((src + m.r0)[r]);
+  const uint32_t* rpb = reinterpret_cast<const uint32_t*>(S.rp + m.r0);  // low words
+  pf.rp0 = rpb[2u * r];
+  pf.rp1 = rpb[2u * r + 2u];
 }
 
 // The J side of the kernel's arguments, copied from the kernarg segment where
