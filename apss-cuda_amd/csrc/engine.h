@@ -90,7 +90,11 @@ class Arena {
   T* alloc(int64_t count) {
     return static_cast<T*>(raw(count < 1 ? sizeof(T) : (size_t)count * sizeof(T)));
   }
-  void release(void* p);  // free one allocation early
+  // The stream whose work may still use the arena's blocks: the destructor
+  // waits for it before the blocks go back to a pool that other threads share
+  // (hipFree would wait by itself).
+  void set_stream(hipStream_t s) { stream_ = s; }
+  void release(void* p);  // free one allocation early (its users must be done)
   void* detach(void* p);  // hand one allocation to the caller (not freed here)
   int64_t used() const { return used_; }
   int64_t peak() const { return peak_; }
@@ -101,6 +105,7 @@ class Arena {
   int device_;
   int64_t budget_;
   DevicePool* pool_;
+  hipStream_t stream_ = nullptr;
   int64_t used_ = 0, peak_ = 0;
   struct Blk {
     void* p;

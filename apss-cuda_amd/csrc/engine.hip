@@ -82,6 +82,8 @@ DevicePool& run_pool(int device) {
 }
 
 Arena::~Arena() {
+  // after an error, work on the stream may still be running
+  if (pool_ && stream_ && !blocks_.empty()) (void)hipStreamSynchronize(stream_);
   for (auto& b : blocks_) drop(b.p, b.size);
 }
 
@@ -1288,6 +1290,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   } sg{st, own_stream};
   auto t0 = std::chrono::steady_clock::now();
   Arena ar(o->device, o->mem_budget, &run_pool(o->device));
+  ar.set_stream(st);
   // p < 0: autotune.  The split is p = 0 (every feature on the tiered
   // inverted-list / dense-head path; p > 0 adds the reference's forward-list
   // head dot, which loses on Zipf data); the tuner picks the tiering.
@@ -1447,6 +1450,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   StreamScope ss(o->stream, o->device);
   hipStream_t st = ss.s;
   Arena ar(o->device, o->mem_budget, &run_pool(o->device));
+  ar.set_stream(st);
   Timer tm(st);
   tm.start();
   int64_t* drp = ar.alloc<int64_t>(n_all + 1);
