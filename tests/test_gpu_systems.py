@@ -211,3 +211,34 @@ def test_text_file_through_reference_shim(tmp_path):
     count = apss.similarity_join_device(parts)
     exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
     assert count == len(exp) > 0
+
+
+def test_concurrent_runs_share_the_pools():
+    # apss_run from several host threads at once: each call has its own
+    # stream, and the device-block and pinned-host pools are shared.  Every
+    # result must equal the single-threaded one, over repeated calls (blocks
+    # freed by one call are taken by the others).
+    mats = [apss.workload("small", n=n, seed_offset=k) for k, n in enumerate((3000, 5000, 7000))]
+    ref = []
+    for rp, col, val, d in mats:
+        pairs, _ = apss.run(rp, col, val, d, 0.5, 0, sort_output=1)
+        ref.append(pairs)
+    errors = []
+
+    def worker(k):
+        try:
+            rp, col, val, d = mats[k]
+            for _ in range(4):
+                pairs, _ = apss.run(rp, col, val, d, 0.5, 0, sort_output=1)
+                if not np.array_equal(pairs, ref[k]):
+                    errors.append(f"matrix {k}: result differs")
+        except Exception as e:  # reported below
+            errors.append(f"matrix {k}: {e!r}")
+
+    threads = [threading.Thread(target=worker, args=(k % 3,)) for k in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    assert sum(len(r) for r in ref) > 1000
