@@ -470,6 +470,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     meta_next = tile_meta(a.J, J_first + 1 < J_end ? J_first + 1 : J_first);
   }
   int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
+  // the band's MFMA operands stay in registers for the whole item
+  v4i avp[4];
+  if constexpr (DM != DENSE_OFF) {
+    const int wv0 = tid >> 6, ln0 = tid & 63;
+    const int rowA = 16 * (wv0 & 7) + (ln0 & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) avp[q] = __builtin_bit_cast(v4i, dA[dslot(rowA, 4 * q + (ln0 >> 4))]);
+  }
 
   for (int32_t J = J_first; J < J_end; ++J) {
     // per-thread addresses are recomputed from an opaque thread index in every
@@ -639,17 +647,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     // of the tile, blk = 0, 1; in the MFMA output layout lane l holds column
     // l%16 and rows 4*(l/16) + v, v = 0..3, i.e. cell m = 4*blk + v.  The
     // operands: lane l reads row l%16 of the block, chunk 4*s + l/16 of the
-    // head record (s = limb for int8, 32-feature step for BF16).
+    // head record (s = limb for int8, 32-feature step for BF16); the band's
+    // operands were read once per work item (avp).
     const int wv = tl >> 6, ln = tl & 63;
     int32_t dv[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) dv[m] = 0;
     if constexpr (DM != DENSE_OFF) {
       const int r15 = ln & 15, g = ln >> 4;
-      const int rowA = 16 * (wv & 7) + r15;
       v4i av[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) av[q] = __builtin_bit_cast(v4i, dA[dslot(rowA, 4 * q + g)]);
+      for (int q = 0; q < 4; ++q) av[q] = avp[q];
 #pragma unroll
       for (int blk = 0; blk < 2; ++blk) {
         const int rowB = 16 * (2 * (wv >> 3) + blk) + r15;
