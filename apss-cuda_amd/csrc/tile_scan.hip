@@ -563,10 +563,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
 
     lds_barrier();
 
-    // flush the staging buffers once one is half full: every append of the
+    // flush the staging buffers once one is three quarters full: every append of the
     // earlier tile pairs is complete here (the next appends come after the
     // MFMA barrier, which also orders the reset of the counts before them)
-    if (stage.n_out >= obuf_cap<DM>() / 2 || stage.n_band >= bbuf_cap<DM>() / 2) flush_stage(a, tl, stage);
+    if (stage.n_out >= obuf_cap<DM>() * 3 / 4 || stage.n_band >= bbuf_cap<DM>() * 3 / 4)
+      flush_stage(a, tl, stage);
 
     // this thread's first cursor product of tile J: its load completes behind
     // the bitmap-tier products.  It is issued before the prefetch of the next
