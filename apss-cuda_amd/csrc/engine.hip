@@ -878,12 +878,12 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     bcap = std::max<int64_t>(1 << 20, std::min<int64_t>(cap, 1 << 24));
   } else {
     // count-only launch over a sample of the bands (every stride-th band
-    // against its partner tiles in 64-tile chunks: every fourth chunk for the
+    // against its partner tiles in 64-tile chunks: every eighth chunk for the
     // output estimate, all of them for the autotuner, which takes the time of
     // the launch, scaled by tile pairs, as the cost of the configuration)
     const int32_t stride = std::max<int32_t>(1, bands.T / 128);
     std::vector<int4> sitems;
-    const int64_t stp = cut_items(64, stride, rc.tune_only ? 1 : 4, sitems);
+    const int64_t stp = cut_items(64, stride, rc.tune_only ? 1 : 8, sitems);
     int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
     APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
     a.items = sdev;
