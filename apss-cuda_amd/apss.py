@@ -47,9 +47,12 @@ class Result(ctypes.Structure):
                 ("products", c_int64), ("head_pairs", c_int64), ("list_visits", c_int64),
                 ("cursor_products", c_int64), ("mfma_features", c_int64),
                 ("dense_rows", c_int64), ("tile_pairs", c_int64), ("bitmap_features", c_int64),
+                ("band_entries", c_int64), ("band_records", c_int64), ("walk_entries", c_int64),
+                ("walk_scatter_entries", c_int64), ("seg_row_bytes", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
-                ("ms_tune", c_double), ("internal_", c_void_p)]
+                ("ms_tune", c_double), ("ms_queue", c_double), ("ms_handover", c_double),
+                ("internal_", c_void_p)]
 
 
 class GenSpec(ctypes.Structure):

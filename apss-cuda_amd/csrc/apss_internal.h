@@ -167,9 +167,12 @@ struct ScanArgs {
   int32_t rescore_all;
   int32_t n_items;           // work items of this launch
   const int4* items;         // {band I, first J tile, one past the last J tile, unused}
-  unsigned long long* stats; // [0] products [1] head pairs [3] list visits [4] cursor products
-                             // (cells, tile pairs and dense records follow from the work
-                             // items and are counted on the host)
+  unsigned long long* stats; // [0] products [1] head pairs [2] band bitmap-tier entries
+                             // (per item) [3] tile-side list visits [4] cursor products
+                             // [5] band cursor records (per item) [6] count-walk reads
+                             // [8] scatter-walk reads ([7] is the work-item queue); cells,
+                             // tile pairs and dense records follow from the work items and
+                             // are counted on the host
   const double* band_t;      // per band: lowest threshold of its rows (NULL: t)
   const double* row_t;       // per I row: threshold (NULL: the band's)
   double eps_max;            // largest cell error bound (for per-band fast rejects)

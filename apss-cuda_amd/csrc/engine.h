@@ -172,6 +172,8 @@ struct TileSet {
 };
 
 TileSide make_view(const Side& s, const TileSet& ts);
+// Returns every device array of a tile set to its arena (its users must be done).
+void release_tiles(Arena& ar, TileSet& ts);
 
 struct Timer {  // hipEvent pair on a stream
   hipEvent_t a = nullptr, b = nullptr;
@@ -231,6 +233,8 @@ struct ScanOutput {
   int64_t count = 0;
   int64_t band = 0, products = 0, head_pairs = 0, cells = 0, visits = 0, cursor = 0;
   int64_t dense_rows = 0, tile_pairs = 0;
+  int64_t band_entries = 0, band_records = 0, walk_entries = 0, walk_scatter = 0;
+  int64_t seg_row_bytes = 0;
   int32_t n_dense = 0;
   double ms_scan = 0, ms_rescore = 0;
 };
@@ -273,6 +277,9 @@ struct StreamScope {
   ~StreamScope();
 };
 
+// Adds a launch's byte-model counters (band entries, records, walk reads) to a
+// result; the segment-row size is the launch's.
+void copy_model_counters(const ScanOutput& so, apss_result* out);
 // Sorting and copying helpers.
 void sort_pairs(apss_pair* p, int64_t n);
 

@@ -221,6 +221,14 @@ TileSide make_view(const Side& s, const TileSet& ts) {
   return v;
 }
 
+void release_tiles(Arena& ar, TileSet& ts) {
+  void* arrays[] = {ts.tstart, ts.row2tile, ts.bmp, ts.wpre, ts.cbase, ts.cst, ts.dbase,
+                    ts.drow, ts.dval, ts.dfeat, ts.sbase, ts.srec, ts.dseg};
+  for (void* q : arrays)
+    if (q) ar.release(q);
+  ts = TileSet{};
+}
+
 static std::string verr_text(uint32_t e) {
   std::string s;
   if (e & VERR_ROWPTR) s += " row_ptr not monotone or ends wrong;";
@@ -739,7 +747,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   const int32_t q_lo = (int32_t)std::max(-2147483647.0, std::min(2147483647.0, qlo));
 
   const double cells = rc.self ? 0.5 * (double)I.n * (double)(I.n - 1) : (double)I.n * (double)J.n;
-  // counters: [0] output pairs [1] band pairs [2..8] kernel statistics [9] work-item queue
+  // counters: [0] output pairs [1] band pairs [2..8] and [10] kernel statistics [9] work-item
+  // queue
   unsigned long long* ctr = ar.alloc<unsigned long long>(16);
 
   // Work items: band b against a chunk of its partner tiles [J_first, J_end)
@@ -811,6 +820,15 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   const int64_t cprod_cap =
       bands.nS > 0 ? std::max<int64_t>((int64_t)1 << 19, bands.max_s * TILE_ROWS) : 0;
   int2* cprod = ar.alloc<int2>(cprod_cap * grid);
+  // the launch's scratch goes back to the arena as soon as the scan is done
+  // (a caller running several launches in one arena, such as a query over
+  // several corpus segments, then holds one launch's scratch at a time)
+  auto release_scratch = [&]() {
+    ar.release(cprod);
+    ar.release(scs);
+    ar.release(items);
+    ar.release(ctr);
+  };
 
   ScanArgs a{};
   a.I = make_view(I, bands);
@@ -905,6 +923,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       so.ms_scan = sample_ms * scale;
       so.tile_pairs = partner_tiles;
       so.count = (int64_t)((double)h2[0] * scale);
+      release_scratch();
       return so;
     }
     cap = (int64_t)std::min(cells, std::min(room, (double)h2[0] * scale * 1.25 + (double)(1 << 22)));
@@ -1009,11 +1028,17 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.cursor = (int64_t)h[6];
     so.dense_rows = host_dense;
     so.tile_pairs = partner_tiles;
+    so.band_entries = (int64_t)h[4];
+    so.band_records = (int64_t)h[7];
+    so.walk_entries = (int64_t)h[8];
+    so.walk_scatter = (int64_t)h[10];
+    so.seg_row_bytes = 2 * (int64_t)tiles.kds;
     so.band = nband;
     so.n_dense = tr.n_dense;
     so.count = nout;
     so.pairs = out;
     ar.release(band);
+    release_scratch();  // the counters were read back above: the stream is idle
     return so;
   }
   throw Error{APSS_E_NOMEM, "output capacity could not be sized"};
@@ -1021,6 +1046,14 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
 
 // ---------------------------------------------------------------------------
 // run_self
+
+void copy_model_counters(const ScanOutput& so, apss_result* out) {
+  out->band_entries += so.band_entries;
+  out->band_records += so.band_records;
+  out->walk_entries += so.walk_entries;
+  out->walk_scatter_entries += so.walk_scatter;
+  out->seg_row_bytes = so.seg_row_bytes;
+}
 
 void sort_pairs(apss_pair* p, int64_t n) {
   std::sort(p, p + n, [](const apss_pair& x, const apss_pair& y) {
@@ -1341,6 +1374,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->mfma_features = so.n_dense;
   out->dense_rows = so.dense_rows;
   out->tile_pairs = so.tile_pairs;
+  copy_model_counters(so, out);
   tm.start();
   if (o->output_on_device && !o->sort_output) {
     out->pairs = (apss_pair*)ar.detach(so.pairs);
@@ -1567,6 +1601,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     out[k].cursor_products = so.cursor;
     out[k].dense_rows = so.dense_rows;
     out[k].tile_pairs = so.tile_pairs;
+    copy_model_counters(so, &out[k]);
     out[k].ms_upload = ms_upload;
     out[k].ms_index = ms_index;
     out[k].ms_scan = so.ms_scan;

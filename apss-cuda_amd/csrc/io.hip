@@ -2,10 +2,11 @@
 //
 //  - the reference's text format (main.cu:286-323): one row per line, a
 //    leading separator character, then "d v" pairs up to the end of the line.
-//    Every file the reference reads parses to the same rows here; in addition
-//    a line may start directly with a digit, pairs may be separated by any run
-//    of blanks, and "\r\n" line ends are accepted.  Ids need not be sorted in
-//    a line (they are sorted here); a repeated id in one line is an error;
+//    Like the reference (main.cu:301 reads one character before the first
+//    pair), the first character of a line is always taken as the separator,
+//    whatever it is.  In addition pairs may be separated by any run of
+//    blanks, and "\r\n" line ends are accepted.  Ids need not be sorted in a
+//    line (they are sorted here); a repeated id in one line is an error;
 //  - a binary CSR file: "APSSCSR1", int64 n_rows, n_cols, nnz, then row_ptr
 //    (int64), col (int32) and val (float32), little-endian;
 //  - pair files: one "i j score" line per pair (the lines compare() reads,
@@ -110,7 +111,7 @@ int apss_load_text(const char* path, int64_t n_cols, apss_csr* out) {
     if (buf[len - 1] == '\n') buf[--len] = 0;
     const char* s = buf;
     const char* end = buf + len;
-    if (s < end && !(*s >= '0' && *s <= '9')) ++s;  // the leading separator
+    if (s < end) ++s;  // the leading separator, consumed unread (main.cu:301)
     row.clear();
     for (;;) {
       while (s < end && blank(*s)) ++s;

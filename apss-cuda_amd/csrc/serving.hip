@@ -30,7 +30,7 @@
 
 namespace apss {
 
-constexpr size_t MAX_SEGMENTS = 4;
+constexpr size_t MAX_SEGMENTS = 5;  // the base segment and up to four deltas
 
 struct HostCSR {
   std::vector<int64_t> rp{0};
@@ -221,14 +221,17 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
       stats->cursor_products += so.cursor;
       stats->dense_rows += so.dense_rows;
       stats->tile_pairs += so.tile_pairs;
+      copy_model_counters(so, stats);
       stats->ms_index += ms_idx;
       stats->ms_scan += so.ms_scan;
       stats->ms_rescore += so.ms_rescore;
     }
+    // this segment's query-side structures go before the next segment's are built
     ar.release(so.pairs);
     ar.release(dt);
     if (qd.dense) ar.release(qd.dense);
     if (qd.dnorm) ar.release(qd.dnorm);
+    release_tiles(ar, bands);
     if (stats) stats->mfma_features = so.n_dense;
   }
   if (stats) stats->ms_upload = ms_up;
@@ -244,6 +247,7 @@ struct Request {
   int64_t est = 0;          // device bytes reserved at admission
   enum State { QUEUED, RUNNING, DONE } state = QUEUED;
   int rc = APSS_OK;
+  std::string err;          // detail of a failed request (apss_last_error after the wait)
   apss_result res{};
   std::chrono::steady_clock::time_point t_submit, t_done;
 };
@@ -280,7 +284,11 @@ static void worker_loop(Server* s, int slot) {
   if (hipSetDevice(s->idx->device) != hipSuccess ||
       hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
     st = nullptr;
-  DevicePool pool;  // this slot's reusable device blocks
+  // this slot's reusable device blocks, capped at its share of the budget so
+  // idle blocks of one slot never hold memory another slot's admitted work needs
+  const int64_t keep = s->budget > 0 ? std::max<int64_t>(s->budget / std::max(s->slots, 1), (int64_t)1 << 30)
+                                     : (int64_t)RUN_POOL_KEEP;
+  DevicePool pool((size_t)keep);
   for (;;) {
     std::vector<Request*> batch;
     int64_t est = 0;
@@ -329,6 +337,7 @@ static void worker_loop(Server* s, int slot) {
     apss_result stats{};
     int rc = APSS_OK;
     std::string err;
+    const auto t_start = std::chrono::steady_clock::now();
     try {
       if (!st) throw Error{APSS_E_DEVICE, "stream creation failed"};
       if (!live.empty()) query_index(*s->idx, Q, bounds, ts, st, est, 0, pairs, &stats, &pool);
@@ -337,7 +346,9 @@ static void worker_loop(Server* s, int slot) {
       err = e.what;
     } catch (...) {
       rc = APSS_E_DEVICE;
+      err = "unexpected exception";
     }
+    const auto t_scanned = std::chrono::steady_clock::now();
     // split by request: pairs carry packed query rows; one request takes the
     // whole buffer, several are split by a counting pass
     std::vector<apss_pair*> outp(batch.size(), nullptr);
@@ -379,7 +390,11 @@ static void worker_loop(Server* s, int slot) {
       for (size_t k = 0; k < batch.size(); ++k) {
         Request* r = batch[k];
         r->rc = rc;
+        r->err = err;
         r->res = stats;
+        r->res.ms_queue = std::chrono::duration<double, std::milli>(t_start - r->t_submit).count();
+        r->res.ms_handover = std::chrono::duration<double, std::milli>(now - t_scanned).count();
+        r->res.ms_total = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
         r->res.count = rc == APSS_OK ? outn[k] : 0;
         r->res.pairs = rc == APSS_OK ? outp[k] : nullptr;
         if (rc != APSS_OK && outp[k]) host_pairs_free(outp[k]);
@@ -587,6 +602,7 @@ int apss_server_wait(apss_server* s, int64_t ticket, apss_result* out, double* l
   if (latency_ms)
     *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   const int rc = r->rc;
+  if (rc != APSS_OK) g_last_error = r->err;
   if (rc != APSS_OK && out->pairs) {
     host_pairs_free(out->pairs);
     out->pairs = nullptr;

@@ -218,7 +218,8 @@ static_assert(WALK_BATCH % 4 == 0 && WALK_PAD >= WALK_BATCH,
               "a walk step reads up to WALK_BATCH - 1 entries past end - 1");
 
 __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int cur, int end,
-                                           int32_t cs, int32_t ce, uint32_t* bkt) {
+                                           int32_t cs, int32_t ce, uint32_t* bkt,
+                                           unsigned int& n_read) {
   const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
   for (int b = cur & ~3;; b += WALK_BATCH) {
     uint32_t pt[WALK_BATCH];
@@ -232,6 +233,7 @@ __device__ __forceinline__ void walk_count(const uint32_t* __restrict__ pos, int
     for (int q = 0; q < WALK_BATCH; ++q) {
       if (b + q < cur) continue;
       const int32_t t = b + q < end ? (int32_t)(pt[q] >> 8) : INT32_MAX;
+      n_read += (go && b + q < end) ? 1u : 0u;  // list entries this walk reads
       go = go && t < ce;
       if (go) atomicAdd(&bkt[t - cs], 1u);
     }
@@ -245,7 +247,8 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
                                             const float* __restrict__ val, int cur, int end,
                                             int32_t cs, int32_t ce, int il, float x,
                                             uint32_t* bfill, int2* __restrict__ cprod,
-                                            unsigned int& n_prod, int32_t* t_next) {
+                                            unsigned int& n_prod, int32_t* t_next,
+                                            unsigned int& n_read) {
   const uint4* pos4 = reinterpret_cast<const uint4*>(pos);
   const float4* val4 = reinterpret_cast<const float4*>(val);
   int consumed = 0;
@@ -280,6 +283,7 @@ __device__ __forceinline__ int walk_scatter(const uint32_t* __restrict__ pos,
     if (!go) {
       n_prod += (unsigned)consumed;
       *t_next = stop_t;
+      n_read += (unsigned)consumed + (cur + consumed < end ? 1u : 0u);
       return cur + consumed;
     }
   }
@@ -369,6 +373,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ int32_t ch_end_s;
   __shared__ unsigned int rec_next;                  // cursor records handed out so far
   __shared__ unsigned long long acc_products, acc_head, acc_visits, acc_cursor;
+  // entries staged once per work item (band bitmap-tier entries, cursor
+  // records) and cursor-list entries read by the count and scatter walks
+  __shared__ unsigned long long acc_band_e, acc_band_s, acc_walk_c, acc_walk_s;
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
   static_assert(BAND_ROWS == 128 && TILE_ROWS == 64 && SCAN_THREADS == 1024,
                 "MFMA layout: 16 waves x two 16x16 blocks cover the 128 x 64 tile");
@@ -381,6 +388,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   for (int c = tid0; c < TILE_ROWS * TSTRIDE; c += SCAN_THREADS) tile[c] = 0;
   if (tid0 == 0) {
     acc_products = 0; acc_head = 0; acc_visits = 0; acc_cursor = 0;
+    acc_band_e = 0; acc_band_s = 0; acc_walk_c = 0; acc_walk_s = 0;
     stage.n_out = 0;
     stage.n_band = 0;
   }
@@ -451,6 +459,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // row-sorted, so positions are ascending in tile << 8 | row) and its tile
   const int64_t sb = a.I.sbase[I];
   const int nS = (int)(a.I.sbase[I + 1] - sb);
+  if (tid == 0) {
+    acc_band_e += (unsigned long long)a.I.dseg[(int64_t)I * a.I.kds + a.KD];
+    acc_band_s += (unsigned long long)nS;
+  }
   for (int k = tid; k < nS; k += SCAN_THREADS) {
     const int4 r = a.I.srec[sb + k];
     int lo = r.z, hi = r.w;
@@ -498,12 +510,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       bkt[tl] = 0;
       if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; rec_next = 0; }
       __syncthreads();
+      unsigned int n_rd = 0;
       for (;;) {
         const int k = (int)atomicAdd(&rec_next, 1u);
         if (k >= nS) break;
         const int4 st = scs[k];
-        if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt);
+        if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt, n_rd);
       }
+      if (n_rd) atomicAdd(&acc_walk_c, (unsigned long long)n_rd);
       __syncthreads();
       if (tl == 0) rec_next = 0;
       const uint32_t cnt = bkt[tl];
@@ -522,11 +536,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         if (st.z >= ce) continue;
         const int4 r = a.I.srec[sb + k];
         int32_t tn;
-        unsigned int np = 0;
+        unsigned int np = 0, nr = 0;
         const int cur = walk_scatter(a.S.pos, a.S.val, st.x, st.y, ch_lo, ce, r.x,
-                                     __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn);
+                                     __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn, nr);
         scs[k] = make_int4(cur, st.y, tn, 0);
         if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
+        if (nr) atomicAdd(&acc_walk_s, (unsigned long long)nr);
       }
       __syncthreads();
       ch_hi = ce;
@@ -614,7 +629,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
 #pragma unroll
       for (int q = 0; q < Z::IPT / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
-        my_visits += 1u + (((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u);
+        // tile-side list entries: counted once per feature and tile pair
+        my_visits += ((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u;
         const int ib = (e_rf[h + q] >> 12) & 0x3fff;  // band row
         const float u = vI[tl + (h + q) * SCAN_THREADS];
         // four entries per step: their LDS reads issue together, then the atomics
@@ -837,6 +853,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     atomicAdd(&a.stats[1], acc_head);
     atomicAdd(&a.stats[3], acc_visits);
     atomicAdd(&a.stats[4], acc_cursor);
+    atomicAdd(&a.stats[2], acc_band_e);
+    atomicAdd(&a.stats[5], acc_band_s);
+    atomicAdd(&a.stats[6], acc_walk_c);
+    atomicAdd(&a.stats[8], acc_walk_s);
   }
 }
 

@@ -32,11 +32,14 @@ Legs printed on the same JSON line (each can be switched off):
             from {0.6, 0.8}, Poisson arrivals at fractions of the measured
             capacity, 1% of the submitted rows inserted into the index while
             queries run; p50 / p99 request latency (submit -> result);
-  roofline  band-scan kernel: algorithmic bytes (list-entry visits of both
-            tiles for every common feature, 8 B each; cursor-tier products,
-            16 B each; dense head records fetched, 256 B each; pairs leaving
-            the tile, 12 B each) over its hipEvent-timed duration on the
-            stream it runs on, against 8 TB/s;
+  h2h       the headline job from host CSR to host pairs (PCIe included),
+            wall time of two calls;
+  roofline  band-scan kernel: algorithmic bytes (alg_terms below: tile-side
+            list entries per tile pair, band-side entries once per work item,
+            cursor walks and products, dense head records, segment-start rows,
+            pairs out) over its hipEvent-timed duration on the stream it runs
+            on, against 8 TB/s; beside it the rocprofv3 HBM traffic of the same
+            launch (profiles/) and its ratio to the model;
   cpu       the reference algorithm restated for the CPU (oracle/), timed on
             the host cores on a stratified sample of query rows, each joined
             with every later row (pair-space throughput).
@@ -60,7 +63,7 @@ import numpy as np  # noqa: E402
 import apss  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-ALL_LEGS = ("t09", "variants", "split", "mixed", "batched", "serving", "cpu")
+ALL_LEGS = ("t09", "variants", "h2h", "split", "mixed", "batched", "serving", "cpu")
 
 
 def log(msg):
@@ -68,19 +71,49 @@ def log(msg):
 
 
 def traffic_from_profiles():
-    """HBM bytes per band-scan launch from the committed rocprofv3 PMC summary, if any."""
+    """The committed rocprofv3 PMC summary of the band scan (FETCH_SIZE and
+    WRITE_SIZE per dispatch, raw and scaled by a copy kernel of known size), or
+    None."""
     path = os.path.join(ROOT, "profiles", "band_scan_large_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f)
     except (OSError, ValueError):
         return None
 
 
+def alg_terms(info):
+    """Algorithmic bytes of one band-scan launch, term by term, from its counters.
+
+    Every operand fetched from HBM counts once per logical use; data the kernel
+    stages in registers or LDS for a whole work item counts once per item.
+      tile_lists    8 B (row id + value) per tile-side list entry: the J tile's
+                    segment of every feature the band holds, per tile pair
+      band_lists    8 B per band-side bitmap-tier entry, once per work item
+      band_records  16 B per band cursor record, once per work item
+      walks         4 B per list position read by the counting walks, 8 B
+                    (position + value) per entry read by the scatter walks
+      cursor        16 B per cursor-tier product (bucket write + read)
+      dense         256 B per dense-head record fetched (tile rows per tile
+                    pair, band rows once per item)
+      seg_rows      one J tile's segment-start row per tile pair
+      pairs_out     12 B per pair leaving the tile (output and error band)
+    """
+    return {
+        "tile_lists": 8.0 * info["list_visits"],
+        "band_lists": 8.0 * info["band_entries"],
+        "band_records": 16.0 * info["band_records"],
+        "walks": 4.0 * info["walk_entries"] + 8.0 * info["walk_scatter_entries"],
+        "cursor": 16.0 * info["cursor_products"],
+        "dense": 256.0 * info["dense_rows"],
+        "seg_rows": float(info["seg_row_bytes"]) * info["tile_pairs"],
+        "pairs_out": 12.0 * (info["count"] + info["band"]),
+    }
+
+
 def alg_bytes(info):
-    """Algorithmic bytes of one band-scan launch from its counters."""
-    return (8.0 * info["list_visits"] + 16.0 * info["cursor_products"] + 256.0 * info["dense_rows"]
-            + 12.0 * (info["count"] + info["band"]))
+    """Algorithmic bytes of one band-scan launch (sum of alg_terms)."""
+    return sum(alg_terms(info).values())
 
 
 def cpu_baseline(rp, col, val, d, t, budget_s):
@@ -134,6 +167,28 @@ def cpu_serial(rp, col, val, d, t, budget_s):
                    f"the full inverted index ({pairs:.3e} pair cells of j > i, {cnt} pairs, {sec:.1f} s); "
                    f"serial inverted-index search, FP32"),
     }
+
+
+def host_to_host(rp, col, val, d, t, p, runs=2, **opts):
+    """The headline job from host CSR to host pairs (upload, search, download
+    into host memory): wall time of each of `runs` calls through the C-ABI."""
+    L = apss.lib()
+    X = apss.make_csr(rp, col, val, d)
+    o = apss.make_opts(**opts)
+    walls, info = [], None
+    for _ in range(runs):
+        r = apss.Result()
+        t0 = time.perf_counter()
+        rc = L.apss_run(apss.ctypes.byref(X), float(t), int(p), apss.ctypes.byref(o), apss.ctypes.byref(r))
+        walls.append((time.perf_counter() - t0) * 1e3)
+        if rc != apss.APSS_OK:
+            raise apss.ApssError(rc, L.apss_last_error().decode())
+        info = info_of(r)
+        L.apss_result_free(apss.ctypes.byref(r))
+    n = len(rp) - 1
+    return {"wall_ms": [round(w, 1) for w in walls], "pairs": info["count"],
+            "pairs_per_s": n * (n - 1) / 2.0 / (min(walls) * 1e-3),
+            "phases_ms": {k: round(info[k], 1) for k in ("ms_upload", "ms_index", "ms_scan", "ms_download")}}
 
 
 def info_of(r):
@@ -271,7 +326,7 @@ def leg_serving(args):
             nreq = int(min(args.serving_max, max(args.serving_min, rate * args.serving_seconds)))
             batches = [make_batch() for _ in range(nreq)]
             gaps = rng.exponential(1.0 / rate, size=nreq)
-            tickets, lats, errs = [], [], []
+            tickets, lats, errs, parts = [], [], [], []
             done = threading.Event()
             cv = threading.Condition()
 
@@ -286,10 +341,12 @@ def leg_serving(args):
                         tk = tickets[k]
                     k += 1
                     try:
-                        _, _, ms = srv.wait_info(tk, keep_pairs=False)
+                        _, info, ms = srv.wait_info(tk, keep_pairs=False)
                         lats.append(ms)
+                        parts.append((info["ms_queue"], ms - info["ms_queue"] - info["ms_handover"],
+                                      info["ms_handover"]))
                     except apss.ApssError as e:
-                        errs.append(e.code)
+                        errs.append({"code": e.code, "detail": str(e)})
 
             th = threading.Thread(target=waiter)
             th.start()
@@ -317,13 +374,29 @@ def leg_serving(args):
             th.join()
             wall = time.perf_counter() - t1
             la = np.array(lats) if lats else np.zeros(1)
+            pa = np.array(parts) if parts else np.zeros((1, 3))
+
+            def pct(a, q):
+                return round(float(np.percentile(a, q)), 2)
+
             sw = {"load": frac, "offered_batches_per_s": round(rate, 2), "requests": nreq, "completed": len(lats),
-                  "errors": len(errs), "p50_ms": round(float(np.percentile(la, 50)), 2),
-                  "p99_ms": round(float(np.percentile(la, 99)), 2), "mean_ms": round(float(la.mean()), 2),
+                  "errors": len(errs), "p50_ms": pct(la, 50), "p99_ms": pct(la, 99), "mean_ms": round(float(la.mean()), 2),
+                  # latency split: queue wait (submit -> launch start), launch (query upload,
+                  # band build, scans and result copy of every corpus segment), hand-over
+                  # (split of a packed batch's pairs by request)
+                  "breakdown_ms": {name: {"p50": pct(pa[:, c], 50), "p99": pct(pa[:, c], 99)}
+                                   for c, name in enumerate(("queue", "launch", "handover"))},
                   "throughput_batches_per_s": round(len(lats) / wall, 2),
                   "query_rows_per_s": round(sub_rows / wall, 1), "inserted_rows": ins_rows, "inserts": inserts}
+            if errs:
+                codes = {}
+                for e in errs:
+                    codes[str(e["code"])] = codes.get(str(e["code"]), 0) + 1
+                sw["error_codes"] = codes
+                sw["error_details"] = sorted({e["detail"] for e in errs})[:4]
             sweeps.append(sw)
-            log(f"serving load {frac}: p50 {sw['p50_ms']} ms p99 {sw['p99_ms']} ms, {sw['throughput_batches_per_s']} batches/s")
+            log(f"serving load {frac}: p50 {sw['p50_ms']} ms p99 {sw['p99_ms']} ms, "
+                f"{sw['throughput_batches_per_s']} batches/s, {len(errs)} errors {sw.get('error_details', '')}")
         out["sweeps"] = sweeps
         out["index_rows_after"] = ix.rows()
     finally:
@@ -399,15 +472,24 @@ def main():
     cells = n * (n - 1) / 2.0
 
     # band-scan roofline: algorithmic bytes / hipEvent kernel time (same stream)
-    ab = alg_bytes(last)
+    terms = alg_terms(last)
+    ab = sum(terms.values())
     scan_ms = float(np.mean([i["ms_scan"] for i in infos]))
     achieved = ab / (scan_ms * 1e-3) / 1e9
+    prof = traffic_from_profiles()
+    traffic = prof.get("hbm_bytes_per_launch") if prof else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_band_scan", "kernel_ms": round(scan_ms, 2), "alg_bytes_per_launch": ab,
-                "terms": {"list_visits": last["list_visits"], "cursor_products": last["cursor_products"],
-                          "dense_rows": last["dense_rows"], "pairs_out": last["count"] + last["band"],
-                          "tile_pairs": last["tile_pairs"]}}
+                "alg_bytes_terms": {k: round(v) for k, v in terms.items()},
+                "counters": {k: last[k] for k in ("list_visits", "band_entries", "band_records", "walk_entries",
+                                                  "walk_scatter_entries", "cursor_products", "dense_rows",
+                                                  "seg_row_bytes", "tile_pairs", "products")},
+                "pairs_out": last["count"] + last["band"]}
+    if prof:
+        roofline["traffic_over_model"] = round(traffic / ab, 3)
+        roofline["traffic_uncalibrated"] = prof.get("raw_bytes_per_launch")
+        roofline["traffic_source"] = "profiles/" + prof.get("source", "band_scan_large_traffic.md")
 
     extra = {}
     if "t09" not in skip:
@@ -427,6 +509,10 @@ def main():
         v["headline_speedup_vs_plain"] = round(v["plain"]["wall_ms"] / ms, 3)
         extra["large_variants"] = v
 
+    if "h2h" not in skip:
+        h2h = host_to_host(rp, col, val, d, args.t, args.p, **opts)
+        extra["host_to_host"] = h2h
+        log(f"host to host: {h2h['wall_ms']} ms, {h2h['pairs']} pairs")
     cpu = None
     if "cpu" not in skip:
         log("cpu baseline ...")

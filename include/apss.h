@@ -109,15 +109,26 @@ typedef struct {
   int64_t band;            /* pairs sent to the FP64 rescore */
   int64_t products;        /* tail partial products accumulated */
   int64_t head_pairs;      /* pairs that ran the head forward-list dot */
-  int64_t list_visits;     /* inverted-list entries visited by tile pairs (both sides) */
+  int64_t list_visits;     /* tile-side inverted-list entries visited: the J tile's segment of
+                              every feature the band holds, once per tile pair */
   int64_t cursor_products; /* products of the short-list (cursor) tier */
   int64_t mfma_features;   /* features in the dense MFMA head (0: none) */
   int64_t dense_rows;      /* 256-byte dense head records fetched by the scan */
   int64_t tile_pairs;      /* band x tile pairs the scan visited */
   int64_t bitmap_features; /* features in the bitmap tier of the tail */
+  int64_t band_entries;    /* band-side bitmap-tier entries, loaded once per work item (they
+                              stay in registers and LDS while the item's tiles pass by) */
+  int64_t band_records;    /* band-side cursor records, loaded once per work item */
+  int64_t walk_entries;    /* cursor-list positions read by the counting walks */
+  int64_t walk_scatter_entries; /* cursor-list entries (position + value) read by the
+                              scatter walks */
+  int64_t seg_row_bytes;   /* bytes of one J tile's segment-start row, fetched per tile pair */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   double ms_tune;          /* autotuner (p < 0): time spent timing candidate configurations */
+  double ms_queue;         /* server requests: submit -> start of the launch that ran it */
+  double ms_handover;      /* server requests: end of the launch -> result ready (split of a
+                              packed batch's pairs by request) */
   void* internal_;
 } apss_result;
 

@@ -3,6 +3,7 @@ rejects bad arguments before touching the device.  No GPU."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -69,4 +70,25 @@ def test_struct_layouts():
     assert ctypes.sizeof(apss.CSR) == 56
     assert ctypes.sizeof(apss.Opts) == 72
     assert apss.PAIR_DTYPE.itemsize == 12
-    assert ctypes.sizeof(apss.Result) == 168
+    assert ctypes.sizeof(apss.Result) == 224
+
+
+def _nm(path, dynamic):
+    args = ["nm", "-D", "--defined-only", path] if dynamic else ["nm", "-u", path]
+    return subprocess.run(args, check=True, capture_output=True, text=True).stdout
+
+
+def test_cxx_linkage_caller_links(tmp_path):
+    # A caller compiled against the reference's declaration (C++ linkage,
+    # similarity_search.h:3-14) references the mangled symbol; libapss.so
+    # defines it beside the C one, so the link succeeds unchanged.
+    src = os.path.join(ROOT, "tests", "caller", "ref_caller.cpp")
+    exe = str(tmp_path / "ref_caller")
+    pkg = os.path.join(ROOT, "apss-cuda_amd")
+    subprocess.run(["g++", "-std=c++17", src, "-L" + pkg, "-lapss", "-Wl,-rpath," + pkg, "-o", exe],
+                   check=True, capture_output=True)
+    undefined = [ln.split()[-1] for ln in _nm(exe, False).splitlines() if "similarity_join_device" in ln]
+    assert undefined and all(u.startswith("_Z22similarity_join_device") for u in undefined)
+    exported = _nm(os.path.join(pkg, "libapss.so"), True)
+    for sym in undefined + ["similarity_join_device", "apss_compat_last_count"]:
+        assert re.search(r"\bT " + re.escape(sym) + r"\b", exported), f"{sym} not exported"

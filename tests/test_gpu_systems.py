@@ -1,5 +1,7 @@
 """Batched runs, the corpus index (queries and inserts), the continuous-batching
 server and the reference-signature shim, each against the FP64 oracle."""
+import os
+import subprocess
 import threading
 
 import numpy as np
@@ -10,6 +12,7 @@ import oracle
 from helpers import check_pairs, subset, tiny_matrix
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def small_rows(n, seed_offset=0, name="small"):
@@ -155,6 +158,40 @@ def test_server_result_same_alone_and_in_the_mix():
         assert lat >= 0
 
 
+def test_server_segments_under_request_budget():
+    # A corpus of a base segment and four delta segments, served with a device
+    # budget that fits about one 4096-row request: each launch holds one
+    # segment's query structures and scan scratch at a time, so every request
+    # completes, and each result equals the oracle's over the whole corpus.
+    crp, ccol, cval, d = small_rows(40_000, name="serving")
+    ix = apss.Index(crp, ccol, cval, d)
+    allr = [(crp, ccol, cval)]
+    try:
+        for k in range(4):
+            rows = small_rows(1500, seed_offset=700 + k, name="serving")[:3]
+            ix.insert(*rows)
+            allr.append(rows)
+        qs = [small_rows(4096, seed_offset=710 + k, name="serving")[:3] for k in range(3)]
+        ts = [0.6, 0.8, 0.6]
+        srv = apss.Server(ix, slots=2, mem_budget=5 << 29)  # 2.5 GiB
+        try:
+            tks = [srv.submit(*q, t) for q, t in zip(qs, ts)]
+            print(f"submitted {len(tks)} requests of 4096 rows", flush=True)
+            got = [srv.wait(tk)[0] for tk in tks]
+        finally:
+            srv.close()
+    finally:
+        ix.close()
+    lens = np.concatenate([np.diff(r[0]) for r in allr])
+    crp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ccol2 = np.concatenate([r[1] for r in allr])
+    cval2 = np.concatenate([r[2] for r in allr])
+    for q, t, g in zip(qs, ts, got):
+        exp = oracle.query_fp64(q, (crp2, ccol2, cval2), d, t)
+        assert len(exp) > 0
+        check_pairs(g, exp)
+
+
 def test_server_cancel_and_budget():
     crp, ccol, cval, d = small_rows(800, name="serving")
     ix = apss.Index(crp, ccol, cval, d)
@@ -209,6 +246,24 @@ def test_text_file_through_reference_shim(tmp_path):
     parts = [((rp[a:b + 1] - rp[a]).astype(np.int64), col[rp[a]:rp[b]].copy(), val[rp[a]:rp[b]].copy())
              for a, b in zip(cuts[:-1], cuts[1:])]
     count = apss.similarity_join_device(parts)
+    exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
+    assert count == len(exp) > 0
+
+
+def test_cxx_caller_binary_counts_pairs(tmp_path):
+    # The reference-style C++ caller (tests/caller/ref_caller.cpp, built by the
+    # package Makefile and linked through the C++-linkage symbol) reads the
+    # reference text format, builds eight SELL-32 partitions and calls
+    # similarity_join_device; its count equals the oracle's at t = 0.3f.
+    exe = os.path.join(ROOT, "apss-cuda_amd", "_build", "ref_caller")
+    assert os.path.exists(exe), "ref_caller not built (run the package Makefile)"
+    rp, col, val, d = small_rows(1800)
+    path = str(tmp_path / "rows.txt")
+    apss.save_text(path, rp, col, val, d)
+    print(f"ref_caller on {path}", flush=True)
+    out = subprocess.run([exe, path], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    count = int(out.stdout.strip().splitlines()[-1].split()[1])
     exp = oracle.apss_fp64(rp, col, val, max(d, 198456), float(np.float32(0.3)))
     assert count == len(exp) > 0
 

@@ -31,9 +31,10 @@ def test_reference_layout(tmp_path):
 
 
 def test_lenient_forms(tmp_path):
-    # Tab separators, CRLF line ends, trailing blanks, no leading separator,
-    # and a last line without a newline.
-    text = "\t2 0.5  4 0.5 \r\n5 0.1\n \r\n 6 2"
+    # Tab separators, CRLF line ends, trailing blanks, a digit as the leading
+    # separator (main.cu:301 consumes the first character of a line whatever it
+    # is, so "15 0.1" is the row {5: 0.1}), and a last line without a newline.
+    text = "\t2 0.5  4 0.5 \r\n15 0.1\n \r\n 6 2"
     rp, col, val, d = apss.load_text(write(tmp_path, "b.txt", text), n_cols=8)
     assert d == 8
     assert rows_of(rp, col, val) == [{2: 0.5, 4: 0.5}, {5: np.float32(0.1)}, {}, {6: 2.0}]
