@@ -64,7 +64,7 @@ class GenSpec(ctypes.Structure):
 # Every symbol declared in include/apss.h.
 ABI_SYMBOLS = [
     "apss_version", "apss_strerror", "apss_last_error", "apss_opts_init", "apss_run",
-    "apss_run_cb", "apss_run_batched", "apss_result_free", "apss_index_create",
+    "apss_run_cb", "apss_run_noalloc", "apss_run_batched", "apss_result_free", "apss_index_create",
     "apss_index_insert", "apss_index_query", "apss_index_rows", "apss_index_destroy",
     "apss_server_create", "apss_server_submit", "apss_server_wait", "apss_server_cancel",
     "apss_server_destroy", "similarity_join_device", "apss_compat_last_count",
@@ -379,6 +379,9 @@ def _bind_more(L):
     L.apss_server_cancel.restype = c_int32
     L.apss_server_destroy.argtypes = [c_void_p]
     L.apss_compat_last_count.restype = c_int64
+    L.apss_run_noalloc.argtypes = [POINTER(CSR), c_double, c_int32, POINTER(Opts), c_void_p, c_int64,
+                                   POINTER(c_int64), POINTER(Result)]
+    L.apss_run_noalloc.restype = c_int32
     L._more_bound = True
     return L
 

@@ -95,6 +95,11 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   });
 }
 
+int apss_run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_pair* pairs,
+                     int64_t capacity, int64_t* count, apss_result* out) {
+  return guarded([&] { return run_noalloc(X, t, p, o, pairs, capacity, count, out); });
+}
+
 int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
                      const apss_opts* o, apss_result* out) {
   return guarded([&] { return run_batched(X, count, t, p, o, out); });

@@ -82,6 +82,12 @@ class Arena {
  public:
   Arena(int device, int64_t budget, DevicePool* pool = nullptr)
       : device_(device), budget_(budget), pool_(pool) {}
+  // Bump mode: every allocation is carved out of [base, base + bytes), memory
+  // the caller owns; nothing is allocated or freed.  release() gives memory
+  // back only from the top (a block released below the top stays taken until
+  // the blocks above it are released too).
+  Arena(int device, void* base, int64_t bytes)
+      : device_(device), budget_(bytes), bump_base_(static_cast<char*>(base)) {}
   int device() const { return device_; }
   ~Arena();
   Arena(const Arena&) = delete;
@@ -98,6 +104,9 @@ class Arena {
   void* detach(void* p);  // hand one allocation to the caller (not freed here)
   int64_t used() const { return used_; }
   int64_t peak() const { return peak_; }
+  // Bytes that can still be allocated under the budget (INT64_MAX: no budget).
+  int64_t room() const { return budget_ > 0 ? budget_ - used_ : INT64_MAX; }
+  bool bump() const { return bump_base_ != nullptr; }
 
  private:
   void* raw(size_t bytes);
@@ -107,10 +116,12 @@ class Arena {
   DevicePool* pool_;
   hipStream_t stream_ = nullptr;
   int64_t used_ = 0, peak_ = 0;
+  char* bump_base_ = nullptr;
   struct Blk {
     void* p;
     size_t size;  // bytes of the block (a pooled block may exceed the request)
     size_t req;   // bytes counted against the budget
+    bool freed = false;  // bump mode: released, waiting for the blocks above it
   };
   std::vector<Blk> blocks_;
 };
@@ -201,6 +212,10 @@ struct RunConfig {
                                        // captured hipGraph
   bool tune_only = false;              // only time the sampled count pass (ms_scan = the
                                        // estimated full scan time)
+  apss_pair* out_given = nullptr;      // output buffer owned by the caller (device): no size
+  int64_t out_given_cap = 0;           // estimate and no regrowth; a larger result is
+                                       // counted (ScanOutput::count) but not all stored
+  int64_t cprod_floor = (int64_t)1 << 19;  // least cursor-product records per workgroup
 };
 
 // Configuration of the tail tiers chosen by the autotuner (p < 0).
@@ -219,7 +234,7 @@ struct Built {
 };
 // Tiers, band and tile tilings, cursor tiles and dense records of one configuration.
 Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                   int32_t dm, const std::vector<int64_t>& bounds);
+                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s = CAP_S);
 // Autotuner: times the sampled count-only scan of each candidate configuration
 // (dense head on / off, bitmap tier at its maximum / 1024 features) and returns
 // the fastest (timed on a prefix of the rows).  `cfg` fills the run
@@ -264,6 +279,9 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
 
 // Full self-join into a result (host or device pairs per opts).
 int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_result* out);
+// Self-join without device allocations (scratch array + caller's output buffer).
+int run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_pair* pairs,
+                int64_t capacity, int64_t* count, apss_result* out);
 // Many independent self-joins in one band-scan launch.
 int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_t p,
                 const apss_opts* o, apss_result* out);

@@ -83,7 +83,8 @@ DevicePool& run_pool(int device) {
 
 Arena::~Arena() {
   // after an error, work on the stream may still be running
-  if (pool_ && stream_ && !blocks_.empty()) (void)hipStreamSynchronize(stream_);
+  if ((pool_ || bump_base_) && stream_ && !blocks_.empty()) (void)hipStreamSynchronize(stream_);
+  if (bump_base_) return;  // the memory belongs to the caller
   for (auto& b : blocks_) drop(b.p, b.size);
 }
 
@@ -94,6 +95,17 @@ void Arena::drop(void* p, size_t bytes) {
 
 void* Arena::raw(size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
+  if (bump_base_) {
+    if (used_ + (int64_t)bytes > budget_)
+      throw Error{APSS_E_NOMEM, "input needs more than the " + std::to_string(budget_) +
+                                    "-byte scratch array (" + std::to_string(used_ + bytes) +
+                                    " bytes so far)"};
+    void* p = bump_base_ + used_;
+    blocks_.push_back({p, bytes, bytes});
+    used_ += (int64_t)bytes;
+    peak_ = std::max(peak_, used_);
+    return p;
+  }
   if (budget_ > 0 && used_ + (int64_t)bytes > budget_)
     throw Error{APSS_E_NOMEM, "device memory budget exceeded (" + std::to_string(used_ + bytes) +
                                   " > " + std::to_string(budget_) + " bytes)"};
@@ -121,6 +133,15 @@ void* Arena::raw(size_t bytes) {
 }
 
 void Arena::release(void* p) {
+  if (bump_base_) {
+    for (auto& b : blocks_)
+      if (b.p == p) b.freed = true;
+    while (!blocks_.empty() && blocks_.back().freed) {
+      used_ -= (int64_t)blocks_.back().req;
+      blocks_.pop_back();
+    }
+    return;
+  }
   for (size_t k = 0; k < blocks_.size(); ++k)
     if (blocks_[k].p == p) {
       drop(p, blocks_[k].size);
@@ -626,7 +647,7 @@ void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
 }
 
 Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                   int32_t dm, const std::vector<int64_t>& bounds) {
+                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s) {
   Built b;
   PhaseTrace ph;
   // with APSS_TRACE set, each step is synchronised so the trace shows its device time
@@ -637,7 +658,7 @@ Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, 
   };
   b.tr = build_tiers(ar, st, s, p, self, kd_cap, dm);
   step("index: tiers + cursor lists");
-  b.bands = build_tiles(ar, st, s, b.tr, self, BAND_ROWS, cap_i_for(b.tr.dm), CAP_S, true, bounds);
+  b.bands = build_tiles(ar, st, s, b.tr, self, BAND_ROWS, cap_i_for(b.tr.dm), cap_s, true, bounds);
   step("index: band blocks");
   b.tiles = build_tiles(ar, st, s, b.tr, self, TILE_ROWS, cap_j_for(b.tr.dm), INT32_MAX, false, bounds);
   step("index: tile blocks");
@@ -814,11 +835,24 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   // queue; each owns a cursor-state region (records of one band) and a
   // cursor-product region large enough for every product one tile pair can
   // receive (records x tile rows)
-  const int grid = std::max(1, std::min(n_items, n_cu));
   const int64_t scs_cap = std::max<int64_t>(bands.max_s, 1);
-  int4* scs = ar.alloc<int4>(scs_cap * grid);
   const int64_t cprod_cap =
-      bands.nS > 0 ? std::max<int64_t>((int64_t)1 << 19, bands.max_s * TILE_ROWS) : 0;
+      bands.nS > 0 ? std::max<int64_t>(rc.cprod_floor, bands.max_s * TILE_ROWS) : 0;
+  // error-band list of a caller-owned output (no estimate pass sizes it)
+  int64_t bcap_given = 0;
+  if (rc.out_given)
+    bcap_given = std::max<int64_t>(1 << 16, std::min<int64_t>(1 << 22, rc.out_given_cap / 16));
+  int grid = std::max(1, std::min(n_items, n_cu));
+  if (ar.bump()) {
+    // a fixed scratch array: as many workgroups as their regions fit beside
+    // the band list (fewer workgroups only lengthen the persistent loop)
+    const int64_t per_wg = scs_cap * (int64_t)sizeof(int4) + cprod_cap * (int64_t)sizeof(int2) + 512;
+    const int64_t left = ar.room() - 8 * bcap_given - ((int64_t)1 << 16);
+    if (left < per_wg)
+      throw Error{APSS_E_NOMEM, "input needs more than the scratch array (scan regions)"};
+    grid = (int)std::min<int64_t>(grid, left / per_wg);
+  }
+  int4* scs = ar.alloc<int4>(scs_cap * grid);
   int2* cprod = ar.alloc<int2>(cprod_cap * grid);
   // the launch's scratch goes back to the arena as soon as the scan is done
   // (a caller running several launches in one arena, such as a query over
@@ -888,7 +922,10 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   APSS_HIP(hipMemGetInfo(&free_b, &total_b));
   const double room = (double)free_b * 0.4 / sizeof(apss_pair);
   int64_t cap, bcap;
-  if (rc.out_cap_hint > 0) {
+  if (rc.out_given) {
+    cap = rc.out_given_cap;
+    bcap = bcap_given;
+  } else if (rc.out_cap_hint > 0) {
     cap = (int64_t)std::min<double>(cells, (double)rc.out_cap_hint);
     bcap = std::max<int64_t>(1 << 20, cap / 64);
   } else if (!rc.tune_only && (cells <= (double)(1 << 24) || bands.T < 64)) {
@@ -933,7 +970,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
                    (double)h2[0] * scale, (double)h2[1] * scale, (long long)stp);
     ph.mark("output estimate");
   }
-  cap = std::max<int64_t>(cap, 1 << 16);
+  if (!rc.out_given) cap = std::max<int64_t>(cap, 1 << 16);
   a.items = items;
   a.n_items = n_items;
 
@@ -960,7 +997,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     if (timed) APSS_HIP(hipEventRecord(ev[2], st));
   };
   for (int attempt = 0; attempt < 3; ++attempt) {
-    apss_pair* out = ar.alloc<apss_pair>(cap);
+    apss_pair* out = rc.out_given ? rc.out_given : ar.alloc<apss_pair>(cap);
     unsigned long long* band = ar.alloc<unsigned long long>(bcap);
     a.out = out;
     a.out_cap = cap;
@@ -1012,12 +1049,21 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     if (trace_on())
       std::fprintf(stderr, "[apss] scan: %lld pairs, %lld band pairs (room %lld, %lld)\n",
                    (long long)nout, (long long)nband, (long long)cap, (long long)bcap);
+    if (rc.out_given && nout > cap && nband <= bcap) {
+      // a caller-owned buffer is never regrown: the count tells the caller
+      // how much room the result needs (the buffer holds `cap` of its pairs)
+      so.count = nout;
+      so.pairs = out;
+      ar.release(band);
+      release_scratch();
+      return so;
+    }
     if (nout > cap || nband > bcap) {
       // capacity too small: every count is exact except the rescore's share of
       // a truncated band list; grow and run again
-      ar.release(out);
+      if (!rc.out_given) ar.release(out);
       ar.release(band);
-      cap = std::max(cap, nout + std::max<int64_t>(0, nband - bcap) + 1024);
+      if (!rc.out_given) cap = std::max(cap, nout + std::max<int64_t>(0, nband - bcap) + 1024);
       bcap = std::max(bcap, nband + 1024);
       continue;
     }

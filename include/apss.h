@@ -161,6 +161,26 @@ int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, i
 
 void apss_result_free(apss_result* r);
 
+/* Device scratch of apss_run_noalloc, per GPU (MiB). */
+#define APSS_NOALLOC_SCRATCH_MIB 512
+
+/* apss_run without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  X must be a
+ * device CSR (on_device = 1); the pairs go to `pairs`, a device buffer of
+ * `capacity` records the caller owns; *count receives the number of result
+ * pairs.  Every other device structure lives in one scratch array of
+ * APSS_NOALLOC_SCRATCH_MIB MiB per GPU, held by the library until the process
+ * ends; an input whose structures do not fit returns APSS_E_NOMEM (the limit is
+ * about 1e5 rows of 40 nonzeros).  A result larger than `capacity` returns
+ * APSS_E_NOMEM with *count set to the room it needs (the buffer then holds an
+ * arbitrary `capacity` of its pairs).  Calls on one GPU share its array and are
+ * serialised by the library.  p >= 0 (no autotuning), no sorting; out
+ * (optional) receives the counters, with pairs = NULL.  Same pair set and
+ * score bounds as apss_run. */
+int apss_run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_pair* pairs,
+                     int64_t capacity, int64_t* count, apss_result* out);
+
 /* Serving: a corpus index resident in device memory. */
 typedef struct apss_index apss_index;
 int apss_index_create(const apss_csr* corpus, int32_t p, const apss_opts* o, apss_index** out);
