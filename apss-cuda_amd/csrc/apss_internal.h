@@ -131,6 +131,17 @@ struct TileSide {
   // dense MFMA head: one 256-byte record per row (16 x 16-byte chunks)
   const uint4* dense = nullptr;
   const float* dnorm = nullptr;   // norm of the head part of each row, rounded up (BF16)
+  const int2* rq = nullptr;       // settle bounds {E, Q} of each row for this launch
+};
+
+// Arguments of k_row_bounds (settle bounds of the rows of one side).
+struct RowBoundArgs {
+  double other_norm;  // largest norm of the other side (rounded up)
+  double qinv, qscale;
+  double e_fixed;     // error terms independent of the rows (dense head)
+  double rel_k;       // relative score tolerance as 1 / tol (<= 0: none)
+  int32_t head;       // split p > 0: add the FP32 head-dot term
+  int32_t pad_;
 };
 
 // Cursor-tier inverted lists of the corpus side, ascending row inside each
@@ -148,11 +159,13 @@ struct ScanArgs {
   CursorLists S;
   int32_t KD, KW, p;
   int32_t self;              // 1: J >= I and only i < j
-  int32_t head_mode;         // 0: no head (p == 0), 1: forward-list dot
+  int32_t head_mode;         // 0: no head (p == 0), 1: forward-list dot (selects the kernel)
   double t;
   // Score tile cells are fixed point: a product x*y adds trunc(x * 2^S * y)
   // (int32).  Partial sums stay below 2^31 because 2^S * |x||y| <= 2^30.
   float qscale;              // 2^S
+  float qinv_f;              // 2^-S (exact in FP32)
+  float cbf_q;               // BF16 head: c_bf 2^S rounded up (per-cell part of the bound)
   int32_t q_lo;              // cells below this cannot reach t (fast reject)
   double qinv;               // 2^-S
   double score_rel;          // emitted scores must be within this relative error

@@ -36,8 +36,10 @@ __global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, c
                              float* dnorm);
 __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
                              const int64_t* tstart, uint32_t* spos, float* sval);
-template <int DM>
+template <int DM, bool HM>
 __global__ void k_band_scan(ScanArgs a);
+__global__ void k_row_bounds(int64_t n, const int64_t* rp, const float* norm, const float* hnorm,
+                             RowBoundArgs b, int2* out);
 __global__ void k_rescore(const unsigned long long* band, const unsigned long long* band_cnt,
                           int64_t band_cap, const int64_t* rpI,
                           const int32_t* colI, const float* valI, const int64_t* rpJ,

@@ -723,6 +723,18 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int3
 // ---------------------------------------------------------------------------
 // scan + rescore
 
+// The band kernel for a dense-head mode; HM: the split p > 0 (prefix-norm
+// filter and forward-list head dot) is compiled in.
+template <bool HM>
+static void launch_band_scan(int32_t dm, int g, hipStream_t st, const ScanArgs& a) {
+  if (dm == DENSE_I8)
+    hipLaunchKernelGGL((k_band_scan<DENSE_I8, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+  else if (dm == DENSE_BF16)
+    hipLaunchKernelGGL((k_band_scan<DENSE_BF16, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_band_scan<DENSE_OFF, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+}
+
 ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
                             const Side& J, const TileSet& tiles, const Tiers& tr,
                             const RunConfig& rc) {
@@ -831,6 +843,28 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   int4* items = ar.alloc<int4>(std::max(n_items, 1));
   if (n_items > 0)
     APSS_HIP(hipMemcpyAsync(items, hitems.data(), sizeof(int4) * n_items, hipMemcpyHostToDevice, st));
+  // settle bounds of the rows of each side (one array when both sides are the
+  // same matrix, as in a self-join)
+  RowBoundArgs rb{};
+  rb.qinv = qinv;
+  rb.qscale = qscale;
+  rb.e_fixed = tr.dm == DENSE_I8 ? eps_dense : tr.dm == DENSE_BF16 ? qinv : 0.0;
+  // emitted FP32 scores within 1e-6 relative: rel_k >= (1 + 2^-24) / (1e-6 - 2^-24)
+  // covers the bound and the final rounding to FP32
+  rb.rel_k = tr.dm == DENSE_BF16 ? 0.0 : 1.07e6;
+  rb.head = tr.p > 0 ? 1 : 0;
+  const bool one_side = I.rp == J.rp && I.n == J.n && I.norm == J.norm;
+  int2* rqI = ar.alloc<int2>(I.n);
+  int2* rqJ = one_side ? rqI : ar.alloc<int2>(J.n);
+  rb.other_norm = mJ;
+  hipLaunchKernelGGL(k_row_bounds, dim3(blocks_for(I.n, 256, 4096)), dim3(256), 0, st, I.n, I.rp, I.norm,
+                     I.hnorm, rb, rqI);
+  if (!one_side) {
+    rb.other_norm = mI;
+    hipLaunchKernelGGL(k_row_bounds, dim3(blocks_for(J.n, 256, 4096)), dim3(256), 0, st, J.n, J.rp,
+                       J.norm, J.hnorm, rb, rqJ);
+  }
+  APSS_HIP(hipGetLastError());
   // persistent grid: one 1024-thread workgroup per CU pulls work items from a
   // queue; each owns a cursor-state region (records of one band) and a
   // cursor-product region large enough for every product one tile pair can
@@ -860,6 +894,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   auto release_scratch = [&]() {
     ar.release(cprod);
     ar.release(scs);
+    if (rqJ != rqI) ar.release(rqJ);
+    ar.release(rqI);
     ar.release(items);
     ar.release(ctr);
   };
@@ -867,6 +903,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   ScanArgs a{};
   a.I = make_view(I, bands);
   a.J = make_view(J, tiles);
+  a.I.rq = rqI;
+  a.J.rq = rqJ;
   a.S.ptr = tr.sptr;
   a.S.val = tr.sval;
   a.S.pos = tr.spos;
@@ -877,9 +915,11 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.head_mode = tr.p > 0 ? 1 : 0;
   a.t = rc.t;
   a.qscale = (float)qscale;
+  a.qinv_f = (float)qinv;
+  a.cbf_q = std::nextafter((float)(c_bf * qscale), INFINITY);
   a.qinv = qinv;
   a.q_lo = q_lo;
-  a.score_rel = 1e-6;
+  a.score_rel = 1e-6;  // (the kernel applies it through the rows' Q bounds)
   a.i_off = rc.i_off;
   a.j_off = rc.j_off;
   a.out_cnt = ctr + 0;
@@ -903,12 +943,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   auto launch = [&](int g) {
     fill_u64(st, ctr, 16, 0);
     if (a.n_items <= 0) return;
-    if (tr.dm == DENSE_I8)
-      hipLaunchKernelGGL(k_band_scan<DENSE_I8>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
-    else if (tr.dm == DENSE_BF16)
-      hipLaunchKernelGGL(k_band_scan<DENSE_BF16>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
-    else
-      hipLaunchKernelGGL(k_band_scan<DENSE_OFF>, dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    if (a.head_mode) launch_band_scan<true>(tr.dm, g, st, a);
+    else launch_band_scan<false>(tr.dm, g, st, a);
     APSS_HIP(hipGetLastError());
   };
   PhaseTrace ph;

@@ -108,8 +108,8 @@ struct Prefetch {
   uint32_t rows[Sizes<DM>::QPT];
   float4 vals[Sizes<DM>::QPT];
   uint4 dense;                    // one 16-byte chunk of the tile's head records
-  uint32_t aux;                   // one row value: norm, head norm or head-part norm
-  uint32_t rp0, rp1;              // low words of row_ptr[r], row_ptr[r+1] (row length)
+  int2 rq;                        // settle bounds {E, Q} of one row
+  uint32_t aux;                   // one row value: head norm (split p > 0) or head-part norm (BF16)
   TileMeta m;
 };
 
@@ -118,7 +118,7 @@ struct Prefetch {
 // loaded value is merged with a default before its use (such merges made the
 // compiler wait for the loads on the spot); validity is applied when the
 // registers are written to LDS.
-template <int DM>
+template <int DM, bool HM>
 __device__ __forceinline__ void prefetch_tile(const TileSide& S, const TileMeta& m, int32_t t,
                                               int tid, Prefetch<DM>& pf) {
   pf.m = m;
@@ -139,16 +139,29 @@ __device__ __forceinline__ void prefetch_tile(const TileSide& S, const TileMeta&
     const uint4* db = S.dense + m.r0 * 16;
     pf.dense = db[(row < m.nrows ? (unsigned)row : 0u) * 16u + (unsigned)(tid & 15)];
   }
-  // per-row values, one per thread of the first four waves (the array is
-  // chosen per wave)
+  // per-row values, one row per lane: the settle bounds, and the head norm
+  // (second wave) or the head-part norm (fourth wave) where the mode uses them
   const int row = tid & 63;
-  const int which = __builtin_amdgcn_readfirstlane(tid >> 6);
   const unsigned r = row < m.nrows ? (unsigned)row : 0u;
-  const float* src = which == 1 ? S.hnorm : (which == 3 && DM == DENSE_BF16) ? S.dnorm : S.norm;
-  pf.aux = __float_as_uint((src + m.r0)[r]);
-  const uint32_t* rpb = reinterpret_cast<const uint32_t*>(S.rp + m.r0);  // low words
-  pf.rp0 = rpb[2u * r];
-  pf.rp1 = rpb[2u * r + 2u];
+  pf.rq = (S.rq + m.r0)[r];
+  if constexpr (HM || DM == DENSE_BF16) {
+    const int which = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const float* src = (HM && which == 1) ? S.hnorm : DM == DENSE_BF16 ? S.dnorm : S.hnorm;
+    pf.aux = __float_as_uint((src + m.r0)[r]);
+  }
+}
+
+// Saturating conversion of an integral double to int32 (thresholds in
+// fixed point: a clamped value only makes a settle decision stricter).
+__device__ __forceinline__ int32_t sat_i32(double x) {
+  return x >= 2147483647.0 ? INT32_MAX : x <= -2147483647.0 ? -2147483647 : (int32_t)x;
+}
+
+// BF16 head: the per-cell part c_bf |x_H| |y_H| of the error bound, in
+// fixed-point units, rounded up (cbf_q = c_bf 2^S, rounded up, with a margin
+// for the two FP32 products).
+__device__ __forceinline__ int32_t bf_cell(float cbf_q, float dn_i, float dn_j) {
+  return (int32_t)ceilf(cbf_q * dn_i * dn_j * (1.0f + 0x1p-20f)) + 1;
 }
 
 // The J side of the kernel's arguments, copied from the kernarg segment where
@@ -350,7 +363,7 @@ __device__ __forceinline__ void stage_append(bool want, const T& rec, T* buf, un
 
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
 // DM selects the dense head (DENSE_OFF, DENSE_I8, DENSE_BF16).
-template <int DM>
+template <int DM, bool HM>
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   using Z = Sizes<DM>;
   __shared__ __align__(16) int32_t tile[TILE_ROWS * TSTRIDE];  // fixed-point scores (x 2^S), transposed
@@ -362,9 +375,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   __shared__ StageBufs<obuf_cap<DM>(), bbuf_cap<DM>()> stage;  // results waiting for a flush
   // per-row values of tile J in two buffers (by J % 2): a wave that settles the
   // cells of tile J may still read them while the next tile is written
-  __shared__ float normI[BAND_ROWS], hnI[BAND_ROWS], normJ[2][TILE_ROWS], hnJ[2][TILE_ROWS];
-  __shared__ int32_t lenI[BAND_ROWS], lenJ[2][TILE_ROWS];
-  __shared__ double tI[BAND_ROWS];                   // threshold of each band row
+  // settle bounds: band rows {E, Q, ceil(t 2^S), floor(t 2^S)}, tile rows {E, Q}
+  // (see k_row_bounds); head norms for the split p > 0
+  __shared__ int4 rqI[BAND_ROWS];
+  __shared__ int2 rqJ[2][TILE_ROWS];
+  __shared__ float hnI[HM ? BAND_ROWS : 1], hnJ[2][HM ? TILE_ROWS : 1];
   __shared__ float vI[Z::CI];                        // band entry values * 2^S, [q][tid]
   __shared__ uint4 dA[DM ? BAND_ROWS * 16 : 1];      // head records of the band
   __shared__ uint4 dB[DM ? TILE_ROWS * 16 : 1];      // head records of tile J
@@ -441,12 +456,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     }
   }
   if (tid < BAND_ROWS) {
-    bool ok = tid < nI;
-    int64_t r = i0 + tid;
-    normI[tid] = ok ? a.I.norm[r] : 0.f;
-    hnI[tid] = ok ? a.I.hnorm[r] : 0.f;
-    lenI[tid] = ok ? (int32_t)(a.I.rp[r + 1] - a.I.rp[r]) : 0;
-    tI[tid] = (ok && a.row_t) ? a.row_t[r] : t_band;
+    const bool ok = tid < nI;
+    const int64_t r = i0 + tid;
+    const int2 rq = ok ? a.I.rq[r] : make_int2(0, 0);
+    // the row's threshold in fixed point (exact: 2^S is a power of two)
+    const double tq = ((ok && a.row_t) ? a.row_t[r] : t_band) * (double)a.qscale;
+    rqI[tid] = make_int4(rq.x, rq.y, sat_i32(ceil(tq)), sat_i32(floor(tq)));
+    if constexpr (HM) hnI[tid] = ok ? a.I.hnorm[r] : 0.f;
     if constexpr (DM == DENSE_BF16) dnI[tid] = ok ? a.I.dnorm[r] : 0.f;
   }
   if constexpr (DM != DENSE_OFF) {  // the band's head records (zero beyond its rows)
@@ -478,7 +494,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   Prefetch<DM> pf;
   TileMeta meta_next{};  // metadata of tile J + 1 while tile J is processed
   if (J_first < J_end) {
-    prefetch_tile<DM>(a.J, tile_meta(a.J, J_first), J_first, tid, pf);
+    prefetch_tile<DM, HM>(a.J, tile_meta(a.J, J_first), J_first, tid, pf);
     meta_next = tile_meta(a.J, J_first + 1 < J_end ? J_first + 1 : J_first);
   }
   int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
@@ -567,11 +583,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     if (tl < 4 * TILE_ROWS) {  // rows past the tile read as zero
       const int row = tl & 63, which = tl >> 6;
       const bool ok = row < nJ;
-      const float v = ok ? __uint_as_float(pf.aux) : 0.f;
-      if (which == 0) normJ[jb][row] = v;
-      else if (which == 1) hnJ[jb][row] = v;
-      else if (which == 2) lenJ[jb][row] = ok ? (int32_t)(pf.rp1 - pf.rp0) : 0;
-      else if (DM == DENSE_BF16) dnJ[jb][row] = v;
+      if (which == 0) rqJ[jb][row] = ok ? pf.rq : make_int2(0, 0);
+      if constexpr (HM)
+        if (which == 1) hnJ[jb][row] = ok ? __uint_as_float(pf.aux) : 0.f;
+      if constexpr (DM == DENSE_BF16)
+        if (which == 3) dnJ[jb][row] = ok ? __uint_as_float(pf.aux) : 0.f;
     }
     if constexpr (DM != DENSE_OFF)
       dB[dslot(tl >> 4, tl & 15)] = (tl >> 4) < nJ ? pf.dense : make_uint4(0, 0, 0, 0);
@@ -604,10 +620,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         // (1.3% faster for the int8 kernel; the BF16 kernel's copy would
         // live in scratch memory, so it reads the hoisted argument)
         const TileSide Js = kargs_J();
-        prefetch_tile<DM>(Js, meta_next, t1, tl, pf);
+        prefetch_tile<DM, HM>(Js, meta_next, t1, tl, pf);
         meta_next = tile_meta(Js, J + 2 < J_end ? J + 2 : J_end - 1);
       } else {
-        prefetch_tile<DM>(a.J, meta_next, t1, tl, pf);
+        prefetch_tile<DM, HM>(a.J, meta_next, t1, tl, pf);
         meta_next = tile_meta(a.J, J + 2 < J_end ? J + 2 : J_end - 1);
       }
     }
@@ -630,7 +646,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int q = 0; q < Z::IPT / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
         // tile-side list entries: counted once per feature and tile pair
-        my_visits += ((e_rf[h + q] >> 26) & 1) ? (unsigned)(y1[q] - y0[q]) : 0u;
+        my_visits += (unsigned)((e_rf[h + q] >> 26) & 1) * (unsigned)(y1[q] - y0[q]);
         const int ib = (e_rf[h + q] >> 12) & 0x3fff;  // band row
         const float u = vI[tl + (h + q) * SCAN_THREADS];
         // four entries per step: their LDS reads issue together, then the atomics
@@ -727,7 +743,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     const int jl_b = 32 * (wv >> 3) + (ln & 15);
 
     // ---- 3. head: prefix-norm filter + forward-list dot -------------------
-    if (a.head_mode) {
+    if constexpr (HM) {
       unsigned int my_head = 0;
       if constexpr (DM != DENSE_OFF) {  // the filter needs the dense part in the tile
 #pragma unroll
@@ -743,13 +759,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         if (il >= nI || jl >= nJ || (a.self && i0 + il >= j0 + jl)) continue;
         int32_t* cell = &tile[jl * TSTRIDE + il];
         const int32_t part = *cell;
-        // pruned when even the Cauchy-Schwarz head bound cannot lift it to t
-        const double bound = (double)part * a.qinv + (double)hnI[il] * (double)hnJ[jb][jl];
-        double eps0 = cell_error_bound(lenI[il] < lenJ[jb][jl] ? lenI[il] : lenJ[jb][jl], normI[il],
-                                       normJ[jb][jl], a.qinv, 0.0);
-        if constexpr (DM == DENSE_I8) eps0 += a.eps_dense;
-        if constexpr (DM == DENSE_BF16) eps0 += a.c_bf * (double)dnI[il] * (double)dnJ[jb][jl] + a.qinv;
-        if (bound + eps0 < tI[il]) continue;
+        // pruned when even the Cauchy-Schwarz head bound, on top of the
+        // partial score and its error bound, cannot lift it to t (floor(t 2^S)
+        // 2^-S <= t, so the test only prunes less)
+        const int4 bi = rqI[il];
+        int64_t e = bi.x < rqJ[jb][jl].x ? bi.x : rqJ[jb][jl].x;
+        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
+        const double bound = (double)((int64_t)part + e) * a.qinv + (double)hnI[il] * (double)hnJ[jb][jl];
+        if (bound < (double)bi.w * a.qinv) continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
         const int64_t ai = a.I.rp[ri], aj = a.J.rp[rj];
         const float h = head_dot(a.I.col + ai, a.I.val + ai, a.I.rp[ri + 1] - ai, a.J.col + aj,
@@ -788,7 +805,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) qmax = dv[m] > qmax ? dv[m] : qmax;
     unsigned cmask = 0;
-    if (__ballot(!plain || qmax >= q_lo)) {
+    if (plain) {
+      // off the self-join diagonal, with q_lo > 0: cells past the band or
+      // tile height hold zero and fail the test by themselves
+      if (__ballot(qmax >= q_lo)) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) cmask |= dv[m] >= q_lo ? (1u << m) : 0u;
+      }
+    } else {
       // valid rows of this thread's two blocks: v < lim[blk] (v = m % 4)
       int lim[2];
 #pragma unroll
@@ -814,19 +838,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int k = 1; k < 8; ++k) q = m == k ? dv[k] : q;
       const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
       bool emit = false, to_band = false;
-      double sd = 0.0;
       if (is_cand) {
-        const int k = lenI[il] < lenJ[jb][jl] ? lenI[il] : lenJ[jb][jl];
-        const double hb = a.head_mode ? gamma32(k + 1) * (double)hnI[il] * (double)hnJ[jb][jl] : 0.0;
-        double eps = cell_error_bound(k, normI[il], normJ[jb][jl], a.qinv, hb);
-        if constexpr (DM == DENSE_I8) eps += a.eps_dense;
-        if constexpr (DM == DENSE_BF16) eps += a.c_bf * (double)dnI[il] * (double)dnJ[jb][jl] + a.qinv;
-        sd = (double)q * a.qinv;
-        const double t_row = tI[il];
-        if (sd - eps >= t_row) {
-          // certain hit; its score is final only if the bound meets the relative tolerance
-          if (a.rescore_all || eps > a.score_rel * (sd - eps)) to_band = true; else emit = true;
-        } else if (sd + eps >= t_row) {
+        // the cell's error bound e (fixed point) is the smaller of its two
+        // rows' bounds; with rqI.z = ceil(t 2^S) and rqI.w = floor(t 2^S):
+        // q - e >= ceil(t 2^S) makes s >= t certain, q + e < floor(t 2^S)
+        // makes s < t certain, and between them the FP64 rescore decides
+        const int4 bi = rqI[il];
+        const int2 bj = rqJ[jb][jl];
+        int64_t e = bi.x < bj.x ? bi.x : bj.x;
+        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
+        if ((int64_t)q - e >= (int64_t)bi.z) {
+          // certain hit; its score is final only if the bound meets the
+          // relative tolerance (q >= the rows' Q, which is e scaled by it)
+          if (a.rescore_all || q < (bi.y < bj.y ? bi.y : bj.y)) to_band = true; else emit = true;
+        } else if ((int64_t)q + e >= (int64_t)bi.w) {
           to_band = true;
         }
       }
@@ -834,7 +859,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       apss_pair pr;
       pr.i = (uint32_t)(i0 + il + a.i_off);
       pr.j = (uint32_t)(j0 + jl + a.j_off);
-      pr.score = (float)sd;
+      pr.score = (float)q * a.qinv_f;  // = (float)(q 2^-S): 2^-S is a power of two
       stage_append(emit, pr, stage.out, obuf_cap<DM>(), &stage.n_out, a.out_cnt, a.out, a.out_cap);
       const unsigned long long key = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
       stage_append(to_band, key, stage.band, bbuf_cap<DM>(), &stage.n_band, a.band_cnt, a.band, a.band_cap);
@@ -860,9 +885,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   }
 }
 
-template __global__ void k_band_scan<DENSE_OFF>(ScanArgs);
-template __global__ void k_band_scan<DENSE_I8>(ScanArgs);
-template __global__ void k_band_scan<DENSE_BF16>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, true>(ScanArgs);
 
 // FP64 rescore of the band: exact products summed in ascending feature id,
 // the same order as the FP64 oracle, so the decision s >= t is settled exactly.
@@ -912,6 +940,36 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
     p.score = (float)s;
     out[slot] = p;
   }
+  }
+}
+
+// Settle bounds of every row of one matrix side for one launch.  E bounds the
+// error of any score-tile cell the row takes part in, in fixed-point units
+// (2^-S), whatever the other row: the cell bound u |x||y| + (k + 2) 2^-S
+// (+ the dense-head and head-dot terms) with the other row's norms replaced by
+// the other side's largest ones and k = min(len_i, len_j) by this row's own
+// length; so the smaller of a cell's two row bounds bounds the cell.  Q is the
+// least fixed-point score at which E meets the relative score tolerance
+// (Q = E (rel_k + 1): q - E >= rel_k E means E <= (q - E) / rel_k), or
+// INT32_MIN when emitted scores carry no relative tolerance (BF16 head).
+__global__ __launch_bounds__(256) void k_row_bounds(int64_t n, const int64_t* __restrict__ rp,
+                                                    const float* __restrict__ norm,
+                                                    const float* __restrict__ hnorm,
+                                                    RowBoundArgs b, int2* __restrict__ out) {
+  const double u = 5.9604644775390625e-08;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t len = rp[r + 1] - rp[r];
+    double eps = u * (double)norm[r] * b.other_norm * (1.0 + 1e-6) + (double)(len + 2) * b.qinv + b.e_fixed;
+    if (b.head) eps += gamma32(len + 1) * (double)hnorm[r] * b.other_norm * (1.0 + 1e-6);
+    const double eq = ceil(eps * b.qscale * (1.0 + 1e-12)) + 1.0;
+    const int32_t E = eq >= 1073741824.0 ? 1073741824 : (int32_t)eq;
+    int32_t Q = INT32_MIN;
+    if (b.rel_k > 0.0) {
+      const double qq = (double)E * (b.rel_k + 1.0);
+      Q = qq >= 2147483647.0 ? INT32_MAX : (int32_t)qq;
+    }
+    out[r] = make_int2(E, Q);
   }
 }
 
