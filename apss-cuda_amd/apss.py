@@ -64,7 +64,7 @@ class GenSpec(ctypes.Structure):
 # Every symbol declared in include/apss.h.
 ABI_SYMBOLS = [
     "apss_version", "apss_strerror", "apss_last_error", "apss_opts_init", "apss_run",
-    "apss_run_cb", "apss_run_noalloc", "apss_run_batched", "apss_result_free", "apss_index_create",
+    "apss_run_cb", "apss_run_noalloc", "apss_transpose", "apss_run_batched", "apss_result_free", "apss_index_create",
     "apss_index_insert", "apss_index_query", "apss_index_rows", "apss_index_destroy",
     "apss_server_create", "apss_server_submit", "apss_server_wait", "apss_server_cancel",
     "apss_server_destroy", "similarity_join_device", "apss_compat_last_count",
@@ -341,6 +341,21 @@ class DeviceCSR:
             self.hip.free(p)
 
 
+def transpose(rp, col, val, d, p=0, **opts):
+    """Inverted index (CSC of the ids >= p) built on the device through
+    apss_transpose: (col_ptr int64[d+1], row int32, val float32)."""
+    L = _bind_more(lib())
+    X = make_csr(rp, col, val, d)
+    o = make_opts(**opts)
+    m = int(np.count_nonzero(np.asarray(col) >= p))
+    cptr = np.zeros(d + 1, np.int64)
+    crow = np.zeros(max(m, 1), np.int32)
+    cval = np.zeros(max(m, 1), np.float32)
+    _check(L.apss_transpose(ctypes.byref(X), int(p), ctypes.byref(o), cptr.ctypes.data,
+                            crow.ctypes.data, cval.ctypes.data))
+    return cptr, crow[:m], cval[:m]
+
+
 def run_csr(X, t, p=0, **opts):
     """apss_run on a prepared CSR struct; returns the Result (caller frees it)."""
     L = lib()
@@ -382,6 +397,8 @@ def _bind_more(L):
     L.apss_run_noalloc.argtypes = [POINTER(CSR), c_double, c_int32, POINTER(Opts), c_void_p, c_int64,
                                    POINTER(c_int64), POINTER(Result)]
     L.apss_run_noalloc.restype = c_int32
+    L.apss_transpose.argtypes = [POINTER(CSR), c_int32, POINTER(Opts), c_void_p, c_void_p, c_void_p]
+    L.apss_transpose.restype = c_int32
     L._more_bound = True
     return L
 

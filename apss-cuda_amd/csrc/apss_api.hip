@@ -95,6 +95,11 @@ int apss_run_cb(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   });
 }
 
+int apss_transpose(const apss_csr* X, int32_t p, const apss_opts* o, int64_t* col_ptr, int32_t* row,
+                   float* val) {
+  return guarded([&] { return run_transpose(X, p, o, col_ptr, row, val); });
+}
+
 int apss_run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_pair* pairs,
                      int64_t capacity, int64_t* count, apss_result* out) {
   return guarded([&] { return run_noalloc(X, t, p, o, pairs, capacity, count, out); });

@@ -261,8 +261,13 @@ void check_csr_host(const apss_csr& X, int32_t p);
 // Upload (if needed) a CSR matrix and validate it; computes norms.
 Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p);
 // Histogram + tier selection + cursor lists of a corpus side.
+// all_lists: every feature >= p with a nonempty list goes to the row-sorted
+// cursor lists (no bitmap tier, no dense head): the plain CSR -> CSC transpose.
 Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                  int32_t dm);
+                  int32_t dm, bool all_lists = false);
+// The transpose of X's features >= p into caller-owned host arrays.
+int run_transpose(const apss_csr* X, int32_t p, const apss_opts* o, int64_t* col_ptr, int32_t* row,
+                  float* val);
 // Dense-head records of a side for the tiers' head features (no-op without a head).
 void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr);
 // Dense-head mode asked for by the options.

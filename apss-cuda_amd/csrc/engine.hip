@@ -333,7 +333,7 @@ int32_t dense_mode(const apss_opts& o) {
 }
 
 Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                  int32_t dm) {
+                  int32_t dm, bool all_lists) {
   Tiers tr;
   tr.d = s.d;
   tr.p = p;
@@ -351,7 +351,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   APSS_HIP(hipStreamSynchronize(st));
   ar.release(cnt);
 
-  const int32_t need = self ? 2 : 1;
+  const int32_t need = self && !all_lists ? 2 : 1;
   // bitmap tier: the lists present in a sizeable share of the 64-row tiles,
   // plus every list longer than one LDS sort piece (the cursor tier would walk
   // them entry by entry), most frequent first, up to the tier's capacity; the
@@ -359,8 +359,9 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   const int64_t T = std::max<int64_t>(1, (s.n + TILE_ROWS - 1) / TILE_ROWS);
   const int64_t lmin = std::max<int64_t>(need, T / 8);
   std::vector<int32_t> cand;
-  for (int64_t c = p; c < d; ++c)
+  for (int64_t c = p; c < d && !all_lists; ++c)
     if (h[c] >= lmin || h[c] > S_LIST_MAX) cand.push_back((int32_t)c);
+  if (all_lists) dm = DENSE_OFF;
   std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) { return h[x] > h[y]; });
   const int32_t slots = dm == DENSE_I8 ? DENSE_I8_FEATURES : dm == DENSE_BF16 ? DENSE_BF16_FEATURES : 0;
   const int32_t n_dense = std::min<int32_t>(slots, (int32_t)cand.size());

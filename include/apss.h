@@ -161,6 +161,16 @@ int apss_run_batched(const apss_csr* const* X, int32_t count, const double* t, i
 
 void apss_result_free(apss_result* r);
 
+/* Inverted index of X: the CSR -> CSC transpose of the features >= p, built on
+ * the device (column histogram, scan, scatter, row sort of every list).
+ * col_ptr[n_cols + 1] (features < p get empty lists), row[m] and val[m] are
+ * caller-owned host arrays, m = the nonzeros with id >= p; rows ascend in each
+ * list.  Replaces generate_index_ptr4 + the host scan + construct_Index2
+ * (similarity_search.cu:105-191, 1091-1263, 221-274), whose lists come out in
+ * arbitrary order. */
+int apss_transpose(const apss_csr* X, int32_t p, const apss_opts* o, int64_t* col_ptr, int32_t* row,
+                   float* val);
+
 /* Device scratch of apss_run_noalloc, per GPU (MiB). */
 #define APSS_NOALLOC_SCRATCH_MIB 512
 

@@ -44,3 +44,67 @@ def check_pairs_abs(got, exp, atol):
     err = np.abs(g["score"].astype(np.float64) - exp["score"])
     assert err.max() <= atol, f"max absolute score error {err.max():.3e} > {atol}"
     return float(err.max())
+
+
+K1 = np.uint64(0x9E3779B97F4A7C15)
+K2 = np.uint64(0xC2B2AE3D27D4EB4F)
+
+
+def sample_rows(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False)).astype(np.int64)
+
+
+def oracle_pairs_touching(rp, col, val, d, sample, t):
+    """Self-join pairs (i < j) touching the sampled rows, from the FP64 oracle
+    (the sample rows queried against all rows): sorted keys i*2^32+j, scores."""
+    q_rp = np.zeros(len(sample) + 1, np.int64)
+    q_rp[1:] = np.cumsum(rp[sample + 1] - rp[sample])
+    q_col = np.concatenate([col[rp[r]:rp[r + 1]] for r in sample])
+    q_val = np.concatenate([val[rp[r]:rp[r + 1]] for r in sample])
+    exp = oracle.query_fp64((q_rp, q_col, q_val), (rp, col, val), d, t)
+    s = sample[exp["i"].astype(np.int64)]
+    c = exp["j"].astype(np.int64)
+    keep = s != c
+    s, c, sc = s[keep], c[keep], exp["score"][keep]
+    key = np.minimum(s, c) * (1 << 32) + np.maximum(s, c)
+    key, first = np.unique(key, return_index=True)
+    return key, sc[first]
+
+
+def stream_touching(rp, col, val, d, sample, t, p=0, **opts):
+    """Runs the self-join through apss_run_cb; returns (count, order-free
+    checksum of all pairs, sorted keys and scores of the pairs touching the
+    sample, counters)."""
+    mask = np.zeros(len(rp) - 1, bool)
+    mask[sample] = True
+    acc = {"count": 0, "sum": 0, "keys": [], "scores": []}
+
+    def emit(chunk):
+        i, j = chunk["i"], chunk["j"]
+        assert np.all(i < j)
+        acc["count"] += len(chunk)
+        h = (i.astype(np.uint64) * K1) ^ (j.astype(np.uint64) * K2)
+        acc["sum"] = (acc["sum"] + int(h.sum(dtype=np.uint64))) % (1 << 64)
+        m = mask[i] | mask[j]
+        if m.any():
+            acc["keys"].append(i[m].astype(np.int64) * (1 << 32) + j[m].astype(np.int64))
+            acc["scores"].append(chunk["score"][m].astype(np.float64))
+
+    info = apss.run_cb(rp, col, val, d, t, p, emit, **opts)
+    assert info["count"] == acc["count"]
+    keys = np.concatenate(acc["keys"]) if acc["keys"] else np.zeros(0, np.int64)
+    scores = np.concatenate(acc["scores"]) if acc["scores"] else np.zeros(0)
+    o = np.argsort(keys)
+    return acc["count"], int(acc["sum"]), keys[o], scores[o], info
+
+
+def check_touching(keys, scores, exp_keys, exp_scores, rel=None, atol=None):
+    assert len(keys) == len(exp_keys), f"sample pairs {len(keys)} != oracle {len(exp_keys)}"
+    assert np.array_equal(keys, exp_keys), "sample pair sets differ"
+    if len(exp_keys):
+        if rel is not None:
+            err = np.abs(scores - exp_scores) / np.abs(exp_scores)
+            assert err.max() <= rel, err.max()
+        if atol is not None:
+            assert np.abs(scores - exp_scores).max() <= atol
+    return len(exp_keys)

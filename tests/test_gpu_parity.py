@@ -164,8 +164,9 @@ def test_invalid_inputs_rejected():
 
 
 def test_dense_output_overflows_buffers():
-    # a very low threshold makes most cells hits: exercises the per-workgroup
-    # output buffer overflow path and the HBM capacity regrowth
+    # a very low threshold makes most cells hits: the LDS staging buffers
+    # overflow into HBM directly (the HBM regrowth rerun is forced in
+    # test_gpu_fullsize.py::test_query_result_outgrows_first_buffer)
     rp, col, val, d = apss.workload("small")
     rp, col, val = subset(rp, col, val, 1200)
     got, exp, info, err = run_and_check(rp, col, val, d, 0.01, 0)
