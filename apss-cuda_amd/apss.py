@@ -38,7 +38,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", c_int32), ("precision", c_int32), ("sort_output", c_int32),
                 ("rescore_all", c_int32), ("output_on_device", c_int32),
                 ("dense_features", c_int32), ("mem_budget", c_int64), ("stream", c_void_p),
-                ("mfma_head", c_int32), ("reserved32_", c_int32), ("reserved_", c_int64 * 3)]
+                ("mfma_head", c_int32), ("band_rows", c_int32), ("reserved_", c_int64 * 3)]
 
 
 class Result(ctypes.Structure):
@@ -51,7 +51,8 @@ class Result(ctypes.Structure):
                 ("walk_scatter_entries", c_int64), ("seg_row_bytes", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
-                ("ms_tune", c_double), ("ms_queue", c_double), ("ms_handover", c_double),
+                ("ms_tune", c_double), ("band_rows", c_int32), ("tile_rows", c_int32),
+                ("items_per_cu", c_int32), ("graph_reused", c_int32), ("ms_queue", c_double), ("ms_handover", c_double),
                 ("internal_", c_void_p)]
 
 

@@ -212,18 +212,23 @@ struct RunConfig {
   int64_t out_cap_hint = 0;            // initial output capacity in pairs (0: from free memory)
   bool use_graph = false;              // run the device job (counters, scan, rescore) as one
                                        // captured hipGraph
-  bool tune_only = false;              // only time the sampled count pass (ms_scan = the
-                                       // estimated full scan time)
+  bool tune_only = false;              // only time a count-only launch (ms_scan = the
+                                       // estimated full scan time; see tune_bands)
+  int32_t tune_bands = 0;              // tune_only: sample this many bands (0: all work items)
+  int32_t items_per_cu = 8;            // work items per CU (sets the partner-tile chunk)
   apss_pair* out_given = nullptr;      // output buffer owned by the caller (device): no size
   int64_t out_given_cap = 0;           // estimate and no regrowth; a larger result is
                                        // counted (ScanOutput::count) but not all stored
   int64_t cprod_floor = (int64_t)1 << 19;  // least cursor-product records per workgroup
 };
 
-// Configuration of the tail tiers chosen by the autotuner (p < 0).
+// Configuration chosen by the autotuner (p < 0).
 struct TuneChoice {
+  int32_t p = 0;           // head / tail split
   int32_t dm = DENSE_OFF;  // dense MFMA head mode
   int32_t kd_cap = 0;      // cap of the bitmap tier (0: the mode's maximum)
+  int32_t band_rows = BAND_ROWS, tile_rows = TILE_ROWS;  // height caps of bands and tiles
+  int32_t items_per_cu = 8;  // work items per CU of the scan's queue
   int32_t candidates = 0;  // configurations timed
   double est_ms = 0.0;     // estimated scan time of the choice
   double ms = 0.0;         // time spent tuning
@@ -236,12 +241,18 @@ struct Built {
 };
 // Tiers, band and tile tilings, cursor tiles and dense records of one configuration.
 Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s = CAP_S);
-// Autotuner: times the sampled count-only scan of each candidate configuration
-// (dense head on / off, bitmap tier at its maximum / 1024 features) and returns
-// the fastest (timed on a prefix of the rows).  `cfg` fills the run
-// configuration of a candidate's tiling.
-TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int32_t self,
+                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s = CAP_S,
+                   int32_t band_rows = BAND_ROWS, int32_t tile_rows = TILE_ROWS);
+// The configuration the options ask for (band / tile height caps), without tuning.
+TuneChoice choice_from_opts(const apss_opts& o);
+// The side's norms and head norms recomputed for the split p (new arrays).
+Side with_split(Arena& ar, hipStream_t st, const Side& s, int32_t p);
+// Autotuner: on a prefix of the rows, times count-only scans of candidate
+// configurations in three stages (the split p on a band sample; the tiering --
+// dense head on / off, bitmap tier at its maximum / 1024 features -- and the
+// band / tile heights; the work items per CU) and returns the fastest.  `cfg`
+// fills the run configuration of a candidate's tiling.
+TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
                     const std::vector<int64_t>& bounds,
                     const std::function<void(RunConfig&, const Built&)>& cfg);
 
@@ -253,6 +264,7 @@ struct ScanOutput {
   int64_t band_entries = 0, band_records = 0, walk_entries = 0, walk_scatter = 0;
   int64_t seg_row_bytes = 0;
   int32_t n_dense = 0;
+  bool graph_reused = false;    // the job ran in the kept graph with updated arguments
   double ms_scan = 0, ms_rescore = 0;
 };
 

@@ -648,7 +648,8 @@ void build_dense(Arena& ar, hipStream_t st, Side& s, const Tiers& tr) {
 }
 
 Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, int32_t kd_cap,
-                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s) {
+                   int32_t dm, const std::vector<int64_t>& bounds, int32_t cap_s, int32_t band_rows,
+                   int32_t tile_rows) {
   Built b;
   PhaseTrace ph;
   // with APSS_TRACE set, each step is synchronised so the trace shows its device time
@@ -659,9 +660,9 @@ Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, 
   };
   b.tr = build_tiers(ar, st, s, p, self, kd_cap, dm);
   step("index: tiers + cursor lists");
-  b.bands = build_tiles(ar, st, s, b.tr, self, BAND_ROWS, cap_i_for(b.tr.dm), cap_s, true, bounds);
+  b.bands = build_tiles(ar, st, s, b.tr, self, band_rows, cap_i_for(b.tr.dm), cap_s, true, bounds);
   step("index: band blocks");
-  b.tiles = build_tiles(ar, st, s, b.tr, self, TILE_ROWS, cap_j_for(b.tr.dm), INT32_MAX, false, bounds);
+  b.tiles = build_tiles(ar, st, s, b.tr, self, tile_rows, cap_j_for(b.tr.dm), INT32_MAX, false, bounds);
   step("index: tile blocks");
   set_cursor_tiles(ar, st, b.tr, b.tiles);
   build_dense(ar, st, s, b.tr);
@@ -669,23 +670,47 @@ Built build_config(Arena& ar, hipStream_t st, Side& s, int32_t p, int32_t self, 
   return b;
 }
 
-TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int32_t self,
+TuneChoice choice_from_opts(const apss_opts& o) {
+  TuneChoice c;
+  if (o.band_rows < 0 || o.band_rows == 1 || o.band_rows > BAND_ROWS)
+    throw Error{APSS_E_INVALID, "band_rows outside 2..128"};
+  if (o.band_rows > 0) {
+    c.band_rows = o.band_rows;
+    c.tile_rows = o.band_rows / 2;
+  }
+  return c;
+}
+
+Side with_split(Arena& ar, hipStream_t st, const Side& s, int32_t p) {
+  Side r = s;
+  r.norm = ar.alloc<float>(s.n);
+  r.hnorm = ar.alloc<float>(s.n);
+  uint32_t* err = ar.alloc<uint32_t>(3);
+  fill_i32(st, (int32_t*)err, 3, 0);
+  if (s.n > 0) {
+    hipLaunchKernelGGL(k_validate_norms, dim3(blocks_for(s.n, 4, 2048)), dim3(256), 0, st, s.n, s.d,
+                       s.nnz, s.rp, s.col, s.val, p, r.norm, r.hnorm, err);
+    APSS_HIP(hipGetLastError());
+  }
+  return r;
+}
+
+TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
                     const std::vector<int64_t>& bounds,
                     const std::function<void(RunConfig&, const Built&)>& cfg) {
   const auto t0 = std::chrono::steady_clock::now();
-  const int32_t head = dense_mode(o);
-  std::vector<std::pair<int32_t, int32_t>> cand;  // {dense mode, bitmap cap}
-  if (head != DENSE_OFF) cand.push_back({head, 0});
-  cand.push_back({DENSE_OFF, 0});
-  if (head != DENSE_OFF) cand.push_back({head, 1024});
-  cand.push_back({DENSE_OFF, 1024});
-  // Candidates are timed on a prefix of the rows (1/16, at least 16384 rows):
-  // the tiers keep their relative shape (the bitmap-tier threshold scales with
-  // the tile count) and each candidate's build costs a fraction of the full one.
+  // Candidates are timed on a prefix of the rows (1/16, at least 16384 and at
+  // most 65536 rows): the tiers keep their relative shape (the bitmap-tier
+  // threshold scales with the tile count) and each candidate's build costs a
+  // fraction of the full one.
   Side sub = s;
   std::vector<int64_t> sbounds;
   if (s.n > 32768) {
-    const int64_t m = std::max<int64_t>(16384, s.n / 16);
+    // the prefix ends at an instance start where one is near (a batched run's
+    // bounds), so no instance is cut short
+    int64_t m = std::min<int64_t>(65536, std::max<int64_t>(16384, s.n / 16));
+    const auto nb = std::lower_bound(bounds.begin(), bounds.end(), m);
+    if (nb != bounds.end() && *nb <= 2 * m) m = *nb;
     int64_t e = 0;
     APSS_HIP(hipMemcpyAsync(&e, s.rp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
@@ -698,22 +723,104 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int3
   }
   TuneChoice best;
   best.est_ms = INFINITY;
-  for (const auto& c : cand) {
-    Arena car(o.device, 0);  // the candidate's structures are freed after timing
-    Side sc = sub;
-    Built b = build_config(car, st, sc, p, self, c.second, c.first, sbounds);
+  // One candidate: build its structures on the prefix (in a pooled arena, so
+  // the candidates reuse each other's blocks; candidates that differ only in
+  // the schedule or in the band sample share one build) and time a count-only
+  // scan; tune_bands > 0 samples that many bands (all their partner tiles), 0
+  // scans the whole prefix.  Returns the time scaled to all of the prefix's
+  // tile pairs and the dense-head mode the build ended with.
+  struct Build {
+    TuneChoice key;
+    std::unique_ptr<Arena> ar;
+    Side side;
+    Built b;
+  };
+  std::vector<std::unique_ptr<Build>> builds;
+  auto same = [](const TuneChoice& x, const TuneChoice& y) {
+    return x.p == y.p && x.dm == y.dm && x.kd_cap == y.kd_cap && x.band_rows == y.band_rows &&
+           x.tile_rows == y.tile_rows;
+  };
+  auto measure = [&](const TuneChoice& c, int32_t tune_bands) {
+    Build* bd = nullptr;
+    for (auto& x : builds)
+      if (same(x->key, c)) bd = x.get();
+    if (!bd) {
+      auto nb = std::make_unique<Build>();
+      nb->key = c;
+      nb->ar = std::make_unique<Arena>(o.device, 0, &run_pool(o.device));
+      nb->ar->set_stream(st);
+      nb->side = c.p > 0 ? with_split(*nb->ar, st, sub, c.p) : sub;
+      nb->b = build_config(*nb->ar, st, nb->side, c.p, self, c.kd_cap, c.dm, sbounds, CAP_S,
+                           c.band_rows, c.tile_rows);
+      bd = nb.get();
+      builds.push_back(std::move(nb));
+    }
     RunConfig rc;
-    cfg(rc, b);
+    cfg(rc, bd->b);
+    rc.p = c.p;
     rc.tune_only = true;
-    ScanOutput so = scan_and_rescore(car, st, sc, b.bands, sc, b.tiles, b.tr, rc);
+    rc.tune_bands = tune_bands;
+    rc.items_per_cu = c.items_per_cu;
+    ScanOutput so = scan_and_rescore(*bd->ar, st, bd->side, bd->b.bands, bd->side, bd->b.tiles, bd->b.tr, rc);
     ++best.candidates;
     if (trace_on())
-      std::fprintf(stderr, "[apss] tune: dense %d bitmap cap %d -> %.2f ms\n", c.first, c.second,
-                   so.ms_scan);
-    if (so.ms_scan < best.est_ms) {
-      best.est_ms = so.ms_scan;
-      best.dm = b.tr.dm;
-      best.kd_cap = c.second;
+      std::fprintf(stderr, "[apss] tune: p %d dense %d bitmap cap %d rows %d/%d items/CU %d -> %.2f ms\n",
+                   c.p, bd->b.tr.dm, c.kd_cap, c.band_rows, c.tile_rows, c.items_per_cu, so.ms_scan);
+    return std::make_pair(so.ms_scan, bd->b.tr.dm);
+  };
+  const int32_t head = dense_mode(o);
+  TuneChoice cur;
+  cur.dm = head;
+  cur.kd_cap = o.dense_features;
+  // 1. the split p (the reference's knob): ids below p take the prefix-norm
+  //    filter and the forward-list head dot.  Timed on a sample of 16 bands,
+  //    since a large p can cost a hundred times the scan.
+  double best_p_ms = INFINITY;
+  for (int32_t p : {0, 1024, 4096}) {
+    if (p > s.d) continue;
+    TuneChoice c = cur;
+    c.p = p;
+    const double ms = measure(c, 16).first;
+    if (ms < best_p_ms) {
+      best_p_ms = ms;
+      cur.p = p;
+    }
+  }
+  // 2. the tiering of the tail and the band / tile heights, on the whole prefix
+  std::vector<TuneChoice> cands;
+  auto add = [&](int32_t dm, int32_t cap, int32_t br, int32_t tr_) {
+    TuneChoice c = cur;
+    c.dm = dm;
+    c.kd_cap = cap;
+    c.band_rows = br;
+    c.tile_rows = tr_;
+    cands.push_back(c);
+  };
+  add(head, cur.kd_cap, BAND_ROWS, TILE_ROWS);
+  if (head != DENSE_OFF) add(DENSE_OFF, cur.kd_cap, BAND_ROWS, TILE_ROWS);
+  add(head, 1024, BAND_ROWS, TILE_ROWS);
+  add(head, cur.kd_cap, BAND_ROWS / 2, TILE_ROWS / 2);
+  for (const TuneChoice& c : cands) {
+    const auto r = measure(c, 0);
+    if (r.first < best.est_ms) {
+      const int32_t cnt = best.candidates;
+      best = c;
+      best.dm = r.second;
+      best.est_ms = r.first;
+      best.candidates = cnt;
+    }
+  }
+  // 3. the schedule: work items per CU (the chunk of partner tiles an item
+  //    covers), which trades load balance against per-item set-up
+  for (int32_t ipc : {4, 16}) {
+    TuneChoice c = best;
+    c.items_per_cu = ipc;
+    const auto r = measure(c, 0);
+    if (r.first < best.est_ms) {
+      const int32_t cnt = best.candidates;
+      best = c;
+      best.est_ms = r.first;
+      best.candidates = cnt;
     }
   }
   APSS_HIP(hipStreamSynchronize(st));
@@ -723,6 +830,57 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t p, int3
 
 // ---------------------------------------------------------------------------
 // scan + rescore
+
+// The band kernel's entry for a dense-head mode and split (graph nodes).
+static void* band_scan_fn(int32_t dm, bool hm) {
+  if (hm)
+    return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, true>)
+         : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, true>)
+                            : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, true>);
+  return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, false>)
+       : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, false>)
+                          : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false>);
+}
+
+namespace {
+// The device job of a batched run as a graph of three kernel nodes (counter
+// clear, band scan, FP64 rescore), kept per device and reused: a later job
+// with the same band kernel updates the nodes' arguments and grids in the
+// instantiated graph (hipGraphExecKernelNodeSetParams) instead of building and
+// instantiating a new one.  Callers hold `mu` from the update until their job
+// has finished on its stream.  No memset node: the counters are cleared by a
+// fill kernel.
+struct JobGraph {
+  std::mutex mu;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t node[3] = {nullptr, nullptr, nullptr};
+  void* scan_fn = nullptr;  // the band kernel the graph holds
+  void build(const hipKernelNodeParams* np) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+    exec = nullptr;
+    g = nullptr;
+    scan_fn = nullptr;
+    APSS_HIP(hipGraphCreate(&g, 0));
+    APSS_HIP(hipGraphAddKernelNode(&node[0], g, nullptr, 0, &np[0]));
+    APSS_HIP(hipGraphAddKernelNode(&node[1], g, &node[0], 1, &np[1]));
+    APSS_HIP(hipGraphAddKernelNode(&node[2], g, &node[1], 1, &np[2]));
+    APSS_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    scan_fn = np[1].func;
+  }
+};
+JobGraph& job_graph(int device) {
+  // created on first use and never destroyed (no destructor runs after the
+  // HIP runtime has shut down)
+  static std::mutex mu;
+  static std::map<int, JobGraph*>* graphs = new std::map<int, JobGraph*>;
+  std::lock_guard<std::mutex> lk(mu);
+  JobGraph*& j = (*graphs)[device];
+  if (!j) j = new JobGraph;
+  return *j;
+}
+}  // namespace
 
 // The band kernel for a dense-head mode; HM: the split p > 0 (prefix-norm
 // filter and forward-list head dot) is compiled in.
@@ -801,6 +959,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       f = (int32_t)(std::upper_bound(tiles.cuts.begin() + 1, tiles.cuts.end(), bands.cuts[b]) -
                     (tiles.cuts.begin() + 1));
     const int32_t e = rc.band_jend.empty() ? tiles.T : rc.band_jend[b];
+    if (e > tiles.T || e < 0)  // the kernel would read tiles that do not exist
+      throw Error{APSS_E_INVALID, "partner tile range past the tiling"};
     jf[b] = f;
     je[b] = e;
     partner_tiles += std::max(0, e - f);
@@ -828,7 +988,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     }
     return tp;
   };
-  const int64_t want_items = 8 * (int64_t)n_cu;
+  const int64_t want_items = (int64_t)std::max(1, rc.items_per_cu) * (int64_t)n_cu;
   const int32_t chunk = (int32_t)std::max<int64_t>(
       16, std::min<int64_t>(4096, (partner_tiles + want_items - 1) / want_items));
   std::vector<int4> hitems;
@@ -965,17 +1125,43 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   } else if (rc.out_cap_hint > 0) {
     cap = (int64_t)std::min<double>(cells, (double)rc.out_cap_hint);
     bcap = std::max<int64_t>(1 << 20, cap / 64);
-  } else if (!rc.tune_only && (cells <= (double)(1 << 24) || bands.T < 64)) {
+  } else if (rc.tune_only) {
+    // the autotuner's measurement: a count-only launch (no output room) over
+    // a sample of tune_bands bands with all their partner tiles, or over the
+    // launch's own work items; the faster of two runs, scaled by tile pairs
+    std::vector<int4> sitems;
+    int64_t stp = partner_tiles;
+    if (rc.tune_bands > 0) stp = cut_items(64, std::max<int32_t>(1, bands.T / rc.tune_bands), 1, sitems);
+    else sitems = hitems;
+    int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
+    APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
+    a.items = sdev;
+    a.n_items = (int32_t)sitems.size();
+    a.out = nullptr;
+    a.out_cap = 0;
+    a.band = nullptr;
+    a.band_cap = 0;
+    double best_ms = INFINITY;
+    for (int rep = 0; rep < 2; ++rep) {
+      Timer ts(st);
+      ts.start();
+      launch(std::max(1, std::min(a.n_items, grid)));
+      best_ms = std::min(best_ms, ts.stop_ms());
+    }
+    ar.release(sdev);
+    so.ms_scan = best_ms * (stp > 0 ? (double)partner_tiles / (double)stp : 1.0);
+    so.tile_pairs = partner_tiles;
+    release_scratch();
+    return so;
+  } else if (cells <= (double)(1 << 24) || bands.T < 64) {
     cap = (int64_t)std::min(cells, room);
     bcap = std::max<int64_t>(1 << 20, std::min<int64_t>(cap, 1 << 24));
   } else {
-    // count-only launch over a sample of the bands (every stride-th band
-    // against its partner tiles in 64-tile chunks: every eighth chunk for the
-    // output estimate, all of them for the autotuner, which takes the time of
-    // the launch, scaled by tile pairs, as the cost of the configuration)
+    // count-only launch over a sample of the bands: every stride-th band
+    // against every eighth 64-tile chunk of its partner tiles
     const int32_t stride = std::max<int32_t>(1, bands.T / 128);
     std::vector<int4> sitems;
-    const int64_t stp = cut_items(64, stride, rc.tune_only ? 1 : 8, sitems);
+    const int64_t stp = cut_items(64, stride, 8, sitems);
     int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
     APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
     a.items = sdev;
@@ -992,14 +1178,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     APSS_HIP(hipMemcpyAsync(h2, ctr, sizeof(h2), hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     ar.release(sdev);
+    (void)sample_ms;
     const double scale = stp > 0 ? (double)partner_tiles / (double)stp : 1.0;
-    if (rc.tune_only) {
-      so.ms_scan = sample_ms * scale;
-      so.tile_pairs = partner_tiles;
-      so.count = (int64_t)((double)h2[0] * scale);
-      release_scratch();
-      return so;
-    }
     cap = (int64_t)std::min(cells, std::min(room, (double)h2[0] * scale * 1.25 + (double)(1 << 22)));
     bcap = (int64_t)((double)h2[1] * scale * 1.25) + (1 << 20);
     if (trace_on())
@@ -1041,28 +1221,53 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     a.band = band;
     a.band_cap = bcap;
     if (rc.use_graph) {
-      hipGraph_t g = nullptr;
-      hipGraphExec_t ge = nullptr;
-      APSS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      try {
-        enqueue(out, cap, band, bcap, false);
-      } catch (...) {
-        (void)hipStreamEndCapture(st, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      APSS_HIP(hipStreamEndCapture(st, &g));
-      struct GraphGuard {
-        hipGraph_t* g;
-        hipGraphExec_t* ge;
-        ~GraphGuard() {
-          if (*ge) (void)hipGraphExecDestroy(*ge);
-          if (*g) (void)hipGraphDestroy(*g);
+      // the job as one graph launch: clear the counters, scan, rescore
+      unsigned long long* c_p = ctr;
+      int64_t c_n = 16;
+      unsigned long long c_v = 0;
+      void* fill_args[] = {&c_p, &c_n, &c_v};
+      void* scan_args[] = {&a};
+      const unsigned long long* r_band = band;
+      const unsigned long long* r_cnt = ctr + 1;
+      int64_t r_bcap = bcap;
+      const int64_t* r_rpI = I.rp;
+      const int32_t* r_colI = I.col;
+      const float* r_valI = I.val;
+      const int64_t* r_rpJ = J.rp;
+      const int32_t* r_colJ = J.col;
+      const float* r_valJ = J.val;
+      double r_t = rc.t;
+      const double* r_rowt = rc.row_t;
+      int64_t r_ioff = rc.i_off, r_joff = rc.j_off;
+      apss_pair* r_out = out;
+      unsigned long long* r_ocnt = ctr + 0;
+      int64_t r_ocap = cap;
+      void* rs_args[] = {&r_band, &r_cnt, &r_bcap, &r_rpI, &r_colI, &r_valI, &r_rpJ, &r_colJ,
+                         &r_valJ, &r_t, &r_rowt, &r_ioff, &r_joff, &r_out, &r_ocnt, &r_ocap};
+      hipKernelNodeParams np[3] = {};
+      np[0].func = reinterpret_cast<void*>(&k_fill_u64);
+      np[0].gridDim = dim3(1);
+      np[0].blockDim = dim3(256);
+      np[0].kernelParams = fill_args;
+      np[1].func = band_scan_fn(tr.dm, a.head_mode != 0);
+      np[1].gridDim = dim3(grid);
+      np[1].blockDim = dim3(SCAN_THREADS);
+      np[1].kernelParams = scan_args;
+      np[2].func = reinterpret_cast<void*>(&k_rescore);
+      np[2].gridDim = dim3(rs_grid);
+      np[2].blockDim = dim3(256);
+      np[2].kernelParams = rs_args;
+      JobGraph& jg = job_graph(dev);
+      std::lock_guard<std::mutex> lk(jg.mu);  // held until the job has finished
+      so.graph_reused = jg.exec && jg.scan_fn == np[1].func;
+      for (int k = 0; k < 3 && so.graph_reused; ++k)
+        if (hipGraphExecKernelNodeSetParams(jg.exec, jg.node[k], &np[k]) != hipSuccess) {
+          (void)hipGetLastError();
+          so.graph_reused = false;
         }
-      } gg{&g, &ge};
-      APSS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      if (!so.graph_reused) jg.build(np);
       APSS_HIP(hipEventRecord(ev[0], st));
-      APSS_HIP(hipGraphLaunch(ge, st));
+      APSS_HIP(hipGraphLaunch(jg.exec, st));
       APSS_HIP(hipEventRecord(ev[2], st));
       APSS_HIP(hipEventSynchronize(ev[2]));
       float ms = 0.f;
@@ -1419,19 +1624,21 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->ms_upload = tm.stop_ms();
   ph.mark("upload + validate");
   int32_t dm = dense_mode(*o), kd_cap = o->dense_features;
+  TuneChoice tc = choice_from_opts(*o);  // when not tuning
   if (tune) {
-    const TuneChoice tc = autotune(st, *o, s, p, 1, {}, [&](RunConfig& r, const Built&) {
+    tc = autotune(st, *o, s, 1, {}, [&](RunConfig& r, const Built&) {
       r.t = t;
-      r.p = p;
       r.self = 1;
     });
+    p = tc.p;
     dm = tc.dm;
     kd_cap = tc.kd_cap;
+    if (p > 0) s = with_split(ar, st, s, p);
     out->ms_tune = tc.ms;
     ph.mark("autotune");
   }
   tm.start();
-  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, {});
+  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, {}, CAP_S, tc.band_rows, tc.tile_rows);
   Tiers& tr = bc.tr;
   TileSet& bands = bc.bands;
   TileSet& tiles = bc.tiles;
@@ -1443,6 +1650,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   rc.p = p;
   rc.self = 1;
   rc.rescore_all = o->rescore_all;
+  rc.items_per_cu = tc.items_per_cu;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   out->ms_scan = so.ms_scan;
   out->ms_rescore = so.ms_rescore;
@@ -1458,6 +1666,9 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   out->dense_rows = so.dense_rows;
   out->tile_pairs = so.tile_pairs;
   copy_model_counters(so, out);
+  out->band_rows = tc.band_rows;
+  out->tile_rows = tc.tile_rows;
+  out->items_per_cu = tc.items_per_cu;
   tm.start();
   if (o->output_on_device && !o->sort_output) {
     out->pairs = (apss_pair*)ar.detach(so.pairs);
@@ -1600,26 +1811,30 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
       const int64_t r = bands.cuts[b];
       const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), r) - off.begin()) - 1;
       const int64_t end_row = off[k + 1];
-      jend[b] = (int32_t)(std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) -
-                          tiles.cuts.begin());
+      // the tiling may cover only a prefix of the rows (the autotuner's), whose
+      // last instance it cuts short: its bands end at the last tile
+      jend[b] = (int32_t)std::min<int64_t>(
+          tiles.T, std::lower_bound(tiles.cuts.begin(), tiles.cuts.end(), end_row) - tiles.cuts.begin());
       if (bt) (*bt)[b] = t[k];
     }
   };
   int32_t dm = dense_mode(*o), kd_cap = o->dense_features;
   double ms_tune = 0.0;
+  TuneChoice tc = choice_from_opts(*o);  // when not tuning
   if (tune) {  // the tuner's candidates decide at the lowest threshold of the batch
-    const TuneChoice tc = autotune(st, *o, s, p, 1, bounds, [&](RunConfig& r, const Built& b) {
+    tc = autotune(st, *o, s, 1, bounds, [&](RunConfig& r, const Built& b) {
       r.t = t_min;
-      r.p = p;
       r.self = 1;
       band_ranges(b.bands, b.tiles, r.band_jend, nullptr);
     });
+    p = tc.p;
     dm = tc.dm;
     kd_cap = tc.kd_cap;
+    if (p > 0) s = with_split(ar, st, s, p);
     ms_tune = tc.ms;
   }
   tm.start();
-  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, bounds);
+  Built bc = build_config(ar, st, s, p, 1, kd_cap, dm, bounds, CAP_S, tc.band_rows, tc.tile_rows);
   Tiers& tr = bc.tr;
   TileSet& bands = bc.bands;
   TileSet& tiles = bc.tiles;
@@ -1647,7 +1862,8 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   rc.band_t = dt;
   rc.row_t = drt;
   rc.t_min = t_min;
-  rc.use_graph = true;  // the batch's device job is one captured graph
+  rc.use_graph = true;  // the batch's device job is one graph launch
+  rc.items_per_cu = tc.items_per_cu;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   // split the pair list by instance
   std::vector<apss_pair> h((size_t)so.count);
@@ -1691,6 +1907,10 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
     out[k].ms_rescore = so.ms_rescore;
     out[k].ms_total = ms_total;
     out[k].ms_tune = ms_tune;
+    out[k].band_rows = tc.band_rows;
+    out[k].tile_rows = tc.tile_rows;
+    out[k].items_per_cu = tc.items_per_cu;
+    out[k].graph_reused = so.graph_reused ? 1 : 0;
     out[k].bitmap_features = tr.n_bitmap;
     out[k].mfma_features = tr.n_dense;
   }

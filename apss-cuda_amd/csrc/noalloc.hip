@@ -108,6 +108,9 @@ int run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
     out->dense_rows = so.dense_rows;
     out->tile_pairs = so.tile_pairs;
     out->bitmap_features = bc.tr.n_bitmap;
+    out->band_rows = BAND_ROWS;
+    out->tile_rows = TILE_ROWS;
+    out->items_per_cu = rc.items_per_cu;
     copy_model_counters(so, out);
     out->ms_scan = so.ms_scan;
     out->ms_rescore = so.ms_rescore;

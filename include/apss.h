@@ -95,7 +95,8 @@ typedef struct {
                               0 = on (64 features as exact int8 limbs in FP32 mode,
                               128 BF16 features in APSS_PREC_BF16_HEAD mode),
                               -1 = off (every feature on the sparse path) */
-  int32_t reserved32_;
+  int32_t band_rows;       /* height cap of the bands, 2..128 (0 = 128); tiles take half;
+                              p < 0 lets the autotuner choose */
   int64_t reserved_[3];
 } apss_opts;
 
@@ -126,6 +127,11 @@ typedef struct {
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   double ms_tune;          /* autotuner (p < 0): time spent timing candidate configurations */
+  int32_t band_rows;       /* height cap of the bands (autotuned or 128) */
+  int32_t tile_rows;       /* height cap of the tiles (autotuned or 64) */
+  int32_t items_per_cu;    /* scan work items per CU (autotuned or 8) */
+  int32_t graph_reused;    /* batched runs: 1 if the device job ran in the library's kept
+                              graph with updated arguments (0: the graph was built) */
   double ms_queue;         /* server requests: submit -> start of the launch that ran it */
   double ms_handover;      /* server requests: end of the launch -> result ready (split of a
                               packed batch's pairs by request) */

@@ -54,28 +54,62 @@ def test_batched_thresholds_are_per_instance():
 
 
 def test_batched_graph_repeated_with_eager_work_between():
-    # every batched call runs its device job as one captured graph; repeat it
-    # with eager runs and synchronisations in between, checking every result
+    # every batched call runs its device job as one graph launch; the graph is
+    # built once and later calls only update its arguments.  Repeat it with
+    # eager runs and synchronisations in between, and with other batches whose
+    # buffers differ, checking every result
     inst = [small_rows(n, seed_offset=50 + k) for k, n in enumerate([900, 400, 1300])]
     ts = [0.5, 0.6, 0.45]
     exp = [oracle.apss_fp64(*x, t) for x, t in zip(inst, ts)]
-    for rep in range(3):
+    other = [small_rows(n, seed_offset=90 + k) for k, n in enumerate([2000, 700])]
+    exp_other = [oracle.apss_fp64(*x, 0.5) for x in other]
+    reused = []
+    for rep in range(4):
         res = apss.run_batched(inst, ts, p=0)
         for (pairs, _), e in zip(res, exp):
             check_pairs(pairs, e)
-        got, _ = apss.run(*inst[rep], ts[rep], 0)  # eager work between the graph launches
-        check_pairs(got, exp[rep])
+        reused.append(res[0][1]["graph_reused"])
+        got, _ = apss.run(*inst[rep % 3], ts[rep % 3], 0)  # eager work between the graph launches
+        check_pairs(got, exp[rep % 3])
+        res = apss.run_batched(other, [0.5, 0.5], p=0)  # same graph, other arguments
+        for (pairs, _), e in zip(res, exp_other):
+            check_pairs(pairs, e)
+        reused.append(res[0][1]["graph_reused"])
+    assert all(reused[1:]), reused
 
 
 def test_autotuned_runs_match_oracle():
     rp, col, val, d = small_rows(3000)
     got, info = apss.run(rp, col, val, d, 0.5, -1)
     check_pairs(got, oracle.apss_fp64(rp, col, val, d, 0.5))
-    assert info["p_used"] == 0 and info["ms_tune"] > 0
+    assert info["p_used"] in (0, 1024, 4096) and info["ms_tune"] > 0
     inst = [small_rows(n, seed_offset=60 + k, name="large") for k, n in enumerate([1200, 700])]
     res = apss.run_batched(inst, [0.4, 0.5], p=-1)
     for x, t, (pairs, info) in zip(inst, [0.4, 0.5], res):
         check_pairs(pairs, oracle.apss_fp64(*x, t))
+        assert info["ms_tune"] > 0
+        assert info["p_used"] in (0, 1024, 4096) and info["items_per_cu"] in (4, 8, 16)
+        assert (info["band_rows"], info["tile_rows"]) in ((128, 64), (64, 32))
+
+
+@pytest.mark.parametrize("p,br", [(1024, 64), (4096, 128), (0, 64), (0, 16)])
+def test_tuned_shapes_match_oracle(p, br):
+    # the configurations the autotuner can choose, set by hand: the split
+    # p > 0 and lower band / tile height caps
+    rp, col, val, d = small_rows(2500, seed_offset=7, name="large")
+    got, info = apss.run(rp, col, val, d, 0.5, p, band_rows=br)
+    check_pairs(got, oracle.apss_fp64(rp, col, val, d, 0.5))
+    assert (info["band_rows"], info["tile_rows"]) == (br, br // 2)
+
+
+def test_autotuned_batch_prefix_inside_an_instance():
+    # the autotuner times candidates on a row prefix; with two 40,000-row
+    # instances the prefix (16,384 rows) ends inside the first instance, whose
+    # bands must then stop at the prefix's last tile
+    inst = [small_rows(40_000, seed_offset=120 + k) for k in range(2)]
+    res = apss.run_batched(inst, [0.7, 0.7], p=-1)
+    for x, (pairs, info) in zip(inst, res):
+        check_pairs(pairs, oracle.apss_fp64(*x, 0.7))
         assert info["ms_tune"] > 0
 
 
