@@ -49,6 +49,7 @@ class Result(ctypes.Structure):
                 ("dense_rows", c_int64), ("tile_pairs", c_int64), ("bitmap_features", c_int64),
                 ("band_entries", c_int64), ("band_records", c_int64), ("walk_entries", c_int64),
                 ("walk_scatter_entries", c_int64), ("seg_row_bytes", c_int64),
+                ("tile_pairs_skipped", c_int64), ("tile_pairs_pruned", c_int64),
                 ("ms_upload", c_double), ("ms_index", c_double), ("ms_scan", c_double),
                 ("ms_rescore", c_double), ("ms_download", c_double), ("ms_total", c_double),
                 ("ms_tune", c_double), ("band_rows", c_int32), ("tile_rows", c_int32),

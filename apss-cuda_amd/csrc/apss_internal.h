@@ -132,6 +132,7 @@ struct TileSide {
   const uint4* dense = nullptr;
   const float* dnorm = nullptr;   // norm of the head part of each row, rounded up (BF16)
   const int2* rq = nullptr;       // settle bounds {E, Q} of each row for this launch
+  const float2* l1m = nullptr;    // per tile: largest row L1 norm, largest |value| (rounded up)
 };
 
 // Arguments of k_row_bounds (settle bounds of the rows of one side).
@@ -204,6 +205,9 @@ struct ScanArgs {
   double eps_dense;
   double c_bf;
 };
+// stats[] slots beside the ScanArgs comment above: tile pairs of sparse bands
+// skipped for want of a common feature, and skipped by a block bound
+constexpr int STAT_SKIP_NO_COMMON = 9, STAT_SKIP_BOUND = 10;
 
 // A piece of a cursor list sorted in LDS, and one merge-pass work unit: the
 // sorted runs [a, a+la) and [a+la, a+la+lb) merge into the same positions of

@@ -23,7 +23,7 @@ __global__ void k_sort_small(const int32_t* lists, int32_t nl, const int64_t* sp
 __global__ void k_sort_chunks(const SortChunk* chunks, int2* sent);
 __global__ void k_merge_runs(const MergeTask* tasks, const int2* src, int2* dst);
 __global__ void k_row_counts(int64_t n, const int64_t* rp, const int32_t* col, int32_t p,
-                             const int32_t* fmap, int32_t* dcnt, int32_t* scnt);
+                             const int32_t* fmap, int32_t* dcnt, int32_t* scnt, int32_t* hcnt);
 __global__ void k_tile_count(BuildSide s);
 __global__ void k_scan_i64(int64_t* a, int64_t n);
 __global__ void k_tile_ptr_scan(BuildSide s);
@@ -36,8 +36,9 @@ __global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, c
                              float* dnorm);
 __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
                              const int64_t* tstart, uint32_t* spos, float* sval);
-template <int DM, bool HM>
+template <int DM, bool HM, bool SP>
 __global__ void k_band_scan(ScanArgs a);
+__global__ void k_tile_l1max(const int64_t* tstart, const int64_t* rp, const float* val, float2* out);
 __global__ void k_row_bounds(int64_t n, const int64_t* rp, const float* norm, const float* hnorm,
                              RowBoundArgs b, int2* out);
 __global__ void k_rescore(const unsigned long long* band, const unsigned long long* band_cnt,
@@ -178,10 +179,13 @@ struct TileSet {
   int64_t* sbase = nullptr;
   int4* srec = nullptr;
   uint16_t* dseg = nullptr;      // T x kds segment starts by bitmap index
+  float2* l1m = nullptr;         // T: largest row L1 norm and |value| of each tile
   int32_t kds = 0;
   int64_t nD = 0, nS = 0;
   int64_t max_s = 0;             // most cursor records in one tile
   std::vector<int64_t> cuts;     // host copy of tstart
+  std::vector<uint8_t> rare;     // per tile: its rows hold cursor-tier features only (no
+                                 // bitmap-tier, dense-head or head feature)
 };
 
 TileSide make_view(const Side& s, const TileSet& ts);
@@ -263,6 +267,7 @@ struct ScanOutput {
   int64_t dense_rows = 0, tile_pairs = 0;
   int64_t band_entries = 0, band_records = 0, walk_entries = 0, walk_scatter = 0;
   int64_t seg_row_bytes = 0;
+  int64_t skip_no_common = 0, skip_bound = 0;  // tile pairs of sparse bands skipped
   int32_t n_dense = 0;
   bool graph_reused = false;    // the job ran in the kept graph with updated arguments
   double ms_scan = 0, ms_rescore = 0;

@@ -237,6 +237,7 @@ TileSide make_view(const Side& s, const TileSet& ts) {
   v.srec = ts.srec;
   v.dseg = ts.dseg;
   v.kds = ts.kds;
+  v.l1m = ts.l1m;
   v.dense = s.dense;
   v.dnorm = s.dnorm;
   return v;
@@ -244,7 +245,7 @@ TileSide make_view(const Side& s, const TileSet& ts) {
 
 void release_tiles(Arena& ar, TileSet& ts) {
   void* arrays[] = {ts.tstart, ts.row2tile, ts.bmp, ts.wpre, ts.cbase, ts.cst, ts.dbase,
-                    ts.drow, ts.dval, ts.dfeat, ts.sbase, ts.srec, ts.dseg};
+                    ts.drow, ts.dval, ts.dfeat, ts.sbase, ts.srec, ts.dseg, ts.l1m};
   for (void* q : arrays)
     if (q) ar.release(q);
   ts = TileSet{};
@@ -515,20 +516,25 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   TileSet ts;
   std::vector<int64_t> cuts(1, 0);
   if (s.n > 0) {
-    int32_t* cnts = ar.alloc<int32_t>(2 * s.n);
+    int32_t* cnts = ar.alloc<int32_t>(3 * s.n);
     hipLaunchKernelGGL(k_row_counts, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n, s.rp,
-                       s.col, tr.p, tr.fmap, cnts, cnts + s.n);
+                       s.col, tr.p, tr.fmap, cnts, cnts + s.n, cnts + 2 * s.n);
     APSS_HIP(hipGetLastError());
-    std::vector<int32_t> dc((size_t)s.n), sc((size_t)s.n);
+    std::vector<int32_t> dc((size_t)s.n), sc((size_t)s.n), hc((size_t)s.n);
     APSS_HIP(hipMemcpyAsync(dc.data(), cnts, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
     APSS_HIP(hipMemcpyAsync(sc.data(), cnts + s.n, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
+    APSS_HIP(hipMemcpyAsync(hc.data(), cnts + 2 * s.n, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     ar.release(cnts);
     if (!records) std::fill(sc.begin(), sc.end(), 0);
     cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s, bounds);
+    ts.rare.assign(cuts.size() - 1, 1);
     for (size_t k = 0; k + 1 < cuts.size(); ++k) {
       int64_t m = 0;
-      for (int64_t r = cuts[k]; r < cuts[k + 1]; ++r) m += sc[r];
+      for (int64_t r = cuts[k]; r < cuts[k + 1]; ++r) {
+        m += sc[r];
+        if (dc[r] != 0 || hc[r] != 0) ts.rare[k] = 0;
+      }
       ts.max_s = std::max(ts.max_s, m);
     }
   }
@@ -605,7 +611,10 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   ts.dseg = ar.alloc<uint16_t>((int64_t)std::max(T, 1) * ts.kds);
   b.dseg = ts.dseg;
   b.kds = ts.kds;
+  // per-tile L1 x max statistics (block bounds of sparse bands)
+  ts.l1m = ar.alloc<float2>(std::max(T, 1));
   if (T > 0) {
+    hipLaunchKernelGGL(k_tile_l1max, dim3(T), dim3(128), 0, st, ts.tstart, s.rp, s.val, ts.l1m);
     hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_tile_compact, dim3(T), dim3(256), 0, st, b);
     if (records && ts.nS > 0)
@@ -834,17 +843,17 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
 // The band kernel's entry for a dense-head mode and split (graph nodes).
 static void* band_scan_fn(int32_t dm, bool hm) {
   if (hm)
-    return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, true>)
-         : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, true>)
-                            : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, true>);
-  return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, false>)
-       : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, false>)
-                          : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false>);
+    return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, true, false>)
+         : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, true, false>)
+                            : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, true, false>);
+  return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, false, false>)
+       : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, false, false>)
+                          : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, false>);
 }
 
 namespace {
-// The device job of a batched run as a graph of three kernel nodes (counter
-// clear, band scan, FP64 rescore), kept per device and reused: a later job
+// The device job of a batched run as a graph of four kernel nodes (counter
+// clear, band scan, its sparse form, FP64 rescore), kept per device and reused: a later job
 // with the same band kernel updates the nodes' arguments and grids in the
 // instantiated graph (hipGraphExecKernelNodeSetParams) instead of building and
 // instantiating a new one.  Callers hold `mu` from the update until their job
@@ -854,7 +863,7 @@ struct JobGraph {
   std::mutex mu;
   hipGraph_t g = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipGraphNode_t node[3] = {nullptr, nullptr, nullptr};
+  hipGraphNode_t node[4] = {nullptr, nullptr, nullptr, nullptr};
   void* scan_fn = nullptr;  // the band kernel the graph holds
   void build(const hipKernelNodeParams* np) {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -866,6 +875,7 @@ struct JobGraph {
     APSS_HIP(hipGraphAddKernelNode(&node[0], g, nullptr, 0, &np[0]));
     APSS_HIP(hipGraphAddKernelNode(&node[1], g, &node[0], 1, &np[1]));
     APSS_HIP(hipGraphAddKernelNode(&node[2], g, &node[1], 1, &np[2]));
+    APSS_HIP(hipGraphAddKernelNode(&node[3], g, &node[2], 1, &np[3]));
     APSS_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
     scan_fn = np[1].func;
   }
@@ -887,11 +897,11 @@ JobGraph& job_graph(int device) {
 template <bool HM>
 static void launch_band_scan(int32_t dm, int g, hipStream_t st, const ScanArgs& a) {
   if (dm == DENSE_I8)
-    hipLaunchKernelGGL((k_band_scan<DENSE_I8, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_band_scan<DENSE_I8, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
   else if (dm == DENSE_BF16)
-    hipLaunchKernelGGL((k_band_scan<DENSE_BF16, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_band_scan<DENSE_BF16, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL((k_band_scan<DENSE_OFF, HM>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_band_scan<DENSE_OFF, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
 }
 
 ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
@@ -993,7 +1003,20 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       16, std::min<int64_t>(4096, (partner_tiles + want_items - 1) / want_items));
   std::vector<int4> hitems;
   cut_items(chunk, 1, 1, hitems);
+  // Items of sparse bands (rows with cursor-tier features only; see the SP
+  // form of k_band_scan) go to the end of the list and run in their own launch.
+  // All thresholds must be above zero: then a cell without products holds no
+  // result.
+  const bool sparse_ok = q_lo > 0 && !bands.rare.empty();
+  auto order_items = [&](std::vector<int4>& v) -> int32_t {  // returns the dense items' count
+    if (!sparse_ok) return (int32_t)v.size();
+    const auto mid = std::stable_partition(v.begin(), v.end(),
+                                           [&](const int4& it) { return !bands.rare[(size_t)it.x]; });
+    return (int32_t)(mid - v.begin());
+  };
+  int32_t n_dense_items = order_items(hitems);
   const int32_t n_items = (int32_t)hitems.size();
+  const int32_t n_dense_main = n_dense_items;
   int64_t host_cells = 0, host_dense = 0;
   for (const int4& it : hitems) {
     const int64_t i0 = bands.cuts[it.x], i1 = bands.cuts[it.x + 1];
@@ -1101,11 +1124,29 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.c_bf = c_bf;
   // BF16 head: emitted scores carry the stated BF16 bound (pair set exact)
   if (tr.dm == DENSE_BF16) a.score_rel = 1e300;
+  // the two launches of a list of a.n_items items, the first n_dense_items of
+  // them for the band kernel of the launch's mode, the rest for the sparse form
+  auto split_args = [&](ScanArgs& d, ScanArgs& sp) {
+    d = a;
+    d.n_items = n_dense_items;
+    sp = a;
+    sp.items = a.items + n_dense_items;
+    sp.n_items = a.n_items - n_dense_items;
+    sp.item_next = ctr + 13;  // its own queue
+  };
   auto launch = [&](int g) {
     fill_u64(st, ctr, 16, 0);
     if (a.n_items <= 0) return;
-    if (a.head_mode) launch_band_scan<true>(tr.dm, g, st, a);
-    else launch_band_scan<false>(tr.dm, g, st, a);
+    ScanArgs d, sp;
+    split_args(d, sp);
+    if (d.n_items > 0) {
+      const int gd = std::max(1, std::min(g, d.n_items));
+      if (a.head_mode) launch_band_scan<true>(tr.dm, gd, st, d);
+      else launch_band_scan<false>(tr.dm, gd, st, d);
+    }
+    if (sp.n_items > 0)
+      hipLaunchKernelGGL((k_band_scan<DENSE_OFF, false, true>), dim3(std::max(1, std::min(g, sp.n_items))),
+                         dim3(SCAN_THREADS), 0, st, sp);
     APSS_HIP(hipGetLastError());
   };
   PhaseTrace ph;
@@ -1133,6 +1174,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     int64_t stp = partner_tiles;
     if (rc.tune_bands > 0) stp = cut_items(64, std::max<int32_t>(1, bands.T / rc.tune_bands), 1, sitems);
     else sitems = hitems;
+    n_dense_items = order_items(sitems);
     int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
     APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
     a.items = sdev;
@@ -1162,6 +1204,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     const int32_t stride = std::max<int32_t>(1, bands.T / 128);
     std::vector<int4> sitems;
     const int64_t stp = cut_items(64, stride, 8, sitems);
+    n_dense_items = order_items(sitems);
     int4* sdev = ar.alloc<int4>((int64_t)std::max<size_t>(sitems.size(), 1));
     APSS_HIP(hipMemcpyAsync(sdev, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice, st));
     a.items = sdev;
@@ -1190,6 +1233,7 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   if (!rc.out_given) cap = std::max<int64_t>(cap, 1 << 16);
   a.items = items;
   a.n_items = n_items;
+  n_dense_items = n_dense_main;
 
   ph.mark("scan set-up");
   // The device job: clear the counters, band scan, FP64 rescore of the band
@@ -1226,7 +1270,10 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       int64_t c_n = 16;
       unsigned long long c_v = 0;
       void* fill_args[] = {&c_p, &c_n, &c_v};
-      void* scan_args[] = {&a};
+      ScanArgs d, sp;
+      split_args(d, sp);
+      void* scan_args[] = {&d};
+      void* sparse_args[] = {&sp};
       const unsigned long long* r_band = band;
       const unsigned long long* r_cnt = ctr + 1;
       int64_t r_bcap = bcap;
@@ -1244,23 +1291,28 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       int64_t r_ocap = cap;
       void* rs_args[] = {&r_band, &r_cnt, &r_bcap, &r_rpI, &r_colI, &r_valI, &r_rpJ, &r_colJ,
                          &r_valJ, &r_t, &r_rowt, &r_ioff, &r_joff, &r_out, &r_ocnt, &r_ocap};
-      hipKernelNodeParams np[3] = {};
+      // (a scan node with no items exits at once)
+      hipKernelNodeParams np[4] = {};
       np[0].func = reinterpret_cast<void*>(&k_fill_u64);
       np[0].gridDim = dim3(1);
       np[0].blockDim = dim3(256);
       np[0].kernelParams = fill_args;
       np[1].func = band_scan_fn(tr.dm, a.head_mode != 0);
-      np[1].gridDim = dim3(grid);
+      np[1].gridDim = dim3(std::max(1, std::min(grid, d.n_items)));
       np[1].blockDim = dim3(SCAN_THREADS);
       np[1].kernelParams = scan_args;
-      np[2].func = reinterpret_cast<void*>(&k_rescore);
-      np[2].gridDim = dim3(rs_grid);
-      np[2].blockDim = dim3(256);
-      np[2].kernelParams = rs_args;
+      np[2].func = reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, true>);
+      np[2].gridDim = dim3(std::max(1, std::min(grid, sp.n_items)));
+      np[2].blockDim = dim3(SCAN_THREADS);
+      np[2].kernelParams = sparse_args;
+      np[3].func = reinterpret_cast<void*>(&k_rescore);
+      np[3].gridDim = dim3(rs_grid);
+      np[3].blockDim = dim3(256);
+      np[3].kernelParams = rs_args;
       JobGraph& jg = job_graph(dev);
       std::lock_guard<std::mutex> lk(jg.mu);  // held until the job has finished
       so.graph_reused = jg.exec && jg.scan_fn == np[1].func;
-      for (int k = 0; k < 3 && so.graph_reused; ++k)
+      for (int k = 0; k < 4 && so.graph_reused; ++k)
         if (hipGraphExecKernelNodeSetParams(jg.exec, jg.node[k], &np[k]) != hipSuccess) {
           (void)hipGetLastError();
           so.graph_reused = false;
@@ -1321,6 +1373,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     so.walk_entries = (int64_t)h[8];
     so.walk_scatter = (int64_t)h[10];
     so.seg_row_bytes = 2 * (int64_t)tiles.kds;
+    so.skip_no_common = (int64_t)h[2 + STAT_SKIP_NO_COMMON];
+    so.skip_bound = (int64_t)h[2 + STAT_SKIP_BOUND];
     so.band = nband;
     so.n_dense = tr.n_dense;
     so.count = nout;
@@ -1341,6 +1395,8 @@ void copy_model_counters(const ScanOutput& so, apss_result* out) {
   out->walk_entries += so.walk_entries;
   out->walk_scatter_entries += so.walk_scatter;
   out->seg_row_bytes = so.seg_row_bytes;
+  out->tile_pairs_skipped += so.skip_no_common;
+  out->tile_pairs_pruned += so.skip_bound;
 }
 
 void sort_pairs(apss_pair* p, int64_t n) {

@@ -238,23 +238,29 @@ __global__ __launch_bounds__(256) void k_row_counts(int64_t n, const int64_t* __
                                                     const int32_t* __restrict__ col, int32_t p,
                                                     const int32_t* __restrict__ fmap,
                                                     int32_t* __restrict__ dcnt,
-                                                    int32_t* __restrict__ scnt) {
+                                                    int32_t* __restrict__ scnt,
+                                                    int32_t* __restrict__ hcnt) {
   const int lane = lane_id();
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
-    int dc = 0, sc = 0;
+    int dc = 0, sc = 0, hc = 0;  // bitmap tier, cursor tier, head (ids < p) + dense head
     for (int64_t e = rp[r] + lane; e < rp[r + 1]; e += 64) {
       int32_t c = col[e];
-      if (c < p) continue;
+      if (c < p) {
+        ++hc;
+        continue;
+      }
       int32_t f = fmap[c];
       dc += f >= 0;
       sc += f == F_CURSOR;
+      hc += f <= F_DENSE0;
     }
     for (int o = 32; o > 0; o >>= 1) {
       dc += __shfl_xor(dc, o);
       sc += __shfl_xor(sc, o);
+      hc += __shfl_xor(hc, o);
     }
-    if (lane == 0) { dcnt[r] = dc; scnt[r] = sc; }
+    if (lane == 0) { dcnt[r] = dc; scnt[r] = sc; hcnt[r] = hc; }
   }
 }
 
@@ -538,6 +544,46 @@ __global__ __launch_bounds__(256) void k_row2tile(const int64_t* __restrict__ ts
                                                   int32_t* __restrict__ row2tile) {
   const int t = blockIdx.x;
   for (int64_t r = tstart[t] + threadIdx.x; r < tstart[t + 1]; r += blockDim.x) row2tile[r] = t;
+}
+
+}  // namespace apss
+
+namespace apss {
+
+// Per tile (or band) of a tiling: the largest L1 norm and the largest |value|
+// of its rows, rounded up.  For two blocks I, J every pair (i, j) has
+// |s_ij| <= min(max_I |x|_1 * max_J max|y|, max_I max|x| * max_J |y|_1), the
+// block form of the reference's L1 x max bound (filter_pair2,
+// similarity_search.cu:533-558).
+__global__ __launch_bounds__(128) void k_tile_l1max(const int64_t* __restrict__ tstart,
+                                                    const int64_t* __restrict__ rp,
+                                                    const float* __restrict__ val, float2* out) {
+  __shared__ float sl[128], sm[128];
+  const int t = blockIdx.x;
+  const int64_t r0 = tstart[t], r1 = tstart[t + 1];
+  float l1 = 0.f, mx = 0.f;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    double sum = 0.0;
+    float m = 0.f;
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      const float v = fabsf(val[k]);
+      sum += (double)v;
+      m = fmaxf(m, v);
+    }
+    l1 = fmaxf(l1, (float)sum * (1.0f + 1e-6f));  // covers the rounding to FP32
+    mx = fmaxf(mx, m);
+  }
+  sl[threadIdx.x] = l1;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 64; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sl[threadIdx.x] = fmaxf(sl[threadIdx.x], sl[threadIdx.x + o]);
+      sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[t] = make_float2(sl[0], sm[0]);
 }
 
 }  // namespace apss

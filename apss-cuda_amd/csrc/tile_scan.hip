@@ -363,7 +363,7 @@ __device__ __forceinline__ void stage_append(bool want, const T& rec, T* buf, un
 
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
 // DM selects the dense head (DENSE_OFF, DENSE_I8, DENSE_BF16).
-template <int DM, bool HM>
+template <int DM, bool HM, bool SP>
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   using Z = Sizes<DM>;
   __shared__ __align__(16) int32_t tile[TILE_ROWS * TSTRIDE];  // fixed-point scores (x 2^S), transposed
@@ -391,6 +391,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // entries staged once per work item (band bitmap-tier entries, cursor
   // records) and cursor-list entries read by the count and scatter walks
   __shared__ unsigned long long acc_band_e, acc_band_s, acc_walk_c, acc_walk_s;
+  __shared__ unsigned long long bsum;  // sparse bands: sum of |products| of one tile
   static_assert(CH_TILES == SCAN_THREADS, "one bucket per thread in the chunk scan");
   static_assert(BAND_ROWS == 128 && TILE_ROWS == 64 && SCAN_THREADS == 1024,
                 "MFMA layout: 16 waves x two 16x16 blocks cover the 128 x 64 tile");
@@ -408,6 +409,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     stage.n_band = 0;
   }
   unsigned int my_products = 0, my_visits = 0;
+  unsigned long long n_skip = 0, n_bound = 0;  // sparse-band tile pairs skipped (thread 0)
 
   for (;;) {
   int tid = tid0;
@@ -438,9 +440,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   // index, value * 2^S).  The block is ordered by frequency rank, so
   // the lanes of a wave hold entries of the same or similar features.
   int32_t e_rf[Z::IPT];  // band row << 12 | bitmap index (KD past the block)
-  {
+  const int nE = a.I.dseg[(int64_t)I * a.I.kds + a.KD];  // entries of the band block
+  if constexpr (!SP) {
     const int64_t d0 = a.I.dbase[I];
-    const int nE = a.I.dseg[(int64_t)I * a.I.kds + a.KD];  // entries of the band block
 #pragma unroll
     for (int q = 0; q < Z::IPT; ++q) {
       const int k = tid + q * SCAN_THREADS;
@@ -491,13 +493,227 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   }
   __syncthreads();
 
+  int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
+
+  // ---- 1. cursor tier: bucket the products of the next chunk of tiles ----
+  // Every record walks its row-sorted list up to the chunk end twice: once
+  // to count the products per J tile, once to write them (fixed point, with
+  // their cell) into this workgroup's region, grouped by tile.  Threads take
+  // records one at a time from a shared counter, so long lists spread over
+  // the workgroup.  A chunk whose products exceed the region ends early at a
+  // tile boundary.
+  // The chunk starts at tile Jc; ch_lo / ch_hi receive its range.
+  auto walk_chunk = [&](int32_t Jc, int tl) {
+    ch_lo = Jc;
+    const int32_t ce0 = Jc + CH_TILES < J_end ? Jc + CH_TILES : J_end;
+    bkt[tl] = 0;
+    if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; rec_next = 0; }
+    __syncthreads();
+    unsigned int n_rd = 0;
+    for (;;) {
+      const int k = (int)atomicAdd(&rec_next, 1u);
+      if (k >= nS) break;
+      const int4 st = scs[k];
+      if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt, n_rd);
+    }
+    if (n_rd) atomicAdd(&acc_walk_c, (unsigned long long)n_rd);
+    __syncthreads();
+    if (tl == 0) rec_next = 0;
+    const uint32_t cnt = bkt[tl];
+    uint32_t total;
+    const uint32_t ex = block_exscan(cnt, tl, wsum, &total);
+    bkt[tl] = ex;
+    bfill[tl] = ex;
+    if (tl == 0) bkt[CH_TILES] = total;
+    if ((int64_t)ex + cnt > a.cprod_cap) atomicMin(&ch_end_s, ch_lo + tl);
+    __syncthreads();
+    const int32_t ce = ch_end_s;
+    for (;;) {
+      const int k = (int)atomicAdd(&rec_next, 1u);
+      if (k >= nS) break;
+      const int4 st = scs[k];
+      if (st.z >= ce) continue;
+      const int4 r = a.I.srec[sb + k];
+      int32_t tn;
+      unsigned int np = 0, nr = 0;
+      const int cur = walk_scatter(a.S.pos, a.S.val, st.x, st.y, ch_lo, ce, r.x,
+                                   __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn, nr);
+      scs[k] = make_int4(cur, st.y, tn, 0);
+      if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
+      if (nr) atomicAdd(&acc_walk_s, (unsigned long long)nr);
+    }
+    __syncthreads();
+    ch_hi = ce;
+  };
+
+  // ---- 4. threshold and emission ---------------------------------------------
+  // Each thread reads and clears its eight cells (+ their dense parts) and
+  // settles the ones that can reach t itself, one candidate per lane per
+  // round.  Valid cells: row below the band height, column below the tile
+  // height, and for self-joins i0 + il < j0 + jl.  Emitted pairs go to the
+  // LDS output buffer, cells within the error bound to the band list.  No
+  // barrier follows: the next tile pair's first barrier orders the cleared
+  // cells and the buffer appends before their next use.
+  // (tile J's first row j0, its height nJ, its per-row buffer jb; dv: the
+  // cells' dense-head parts, added to the score tile's)
+  auto settle = [&](int64_t j0, int nJ, int jb, int32_t (&dv)[8], int tl) {
+    const int wv = tl >> 6, ln = tl & 63;
+    const int il_b = 16 * (wv & 7) + 4 * (ln >> 4);
+    const int jl_b = 32 * (wv >> 3) + (ln & 15);
+    const int64_t dj = j0 - i0;
+    // Cells past the band or tile height receive no products and a zero head,
+    // so with q_lo > 0 only the self-join diagonal needs an explicit validity
+    // test: off the diagonal one max over the eight cells rejects them all.
+    const bool plain = q_lo > 0 && (!a.self || dj >= BAND_ROWS);
+    int32_t qmax = INT32_MIN;
+#pragma unroll
+    for (int blk = 0; blk < 2; ++blk) {  // four band rows of one tile row: one 16-byte word
+      int4* w4 = reinterpret_cast<int4*>(&tile[(jl_b + 16 * blk) * TSTRIDE + il_b]);
+      const int4 c4 = *w4;
+      *w4 = make_int4(0, 0, 0, 0);
+      dv[4 * blk + 0] += c4.x;  // full fixed-point score of the cell
+      dv[4 * blk + 1] += c4.y;
+      dv[4 * blk + 2] += c4.z;
+      dv[4 * blk + 3] += c4.w;
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) qmax = dv[m] > qmax ? dv[m] : qmax;
+    unsigned cmask = 0;
+    if (plain) {
+      // off the self-join diagonal, with q_lo > 0: cells past the band or
+      // tile height hold zero and fail the test by themselves
+      if (__ballot(qmax >= q_lo)) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) cmask |= dv[m] >= q_lo ? (1u << m) : 0u;
+      }
+    } else {
+      // valid rows of this thread's two blocks: v < lim[blk] (v = m % 4)
+      int lim[2];
+#pragma unroll
+      for (int blk = 0; blk < 2; ++blk) {
+        const int jl = jl_b + 16 * blk;
+        int l = jl < nJ ? nI - il_b : 0;
+        if (a.self) {  // i0 + il < j0 + jl  <=>  il - il_b < dj + jl - il_b
+          const int64_t dl = dj + jl - il_b;
+          l = dl < l ? (int)(dl < 0 ? 0 : dl) : l;
+        }
+        lim[blk] = l < 0 ? 0 : (l > 4 ? 4 : l);
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        cmask |= ((m & 3) < lim[m >> 2] && dv[m] >= q_lo) ? (1u << m) : 0u;
+    }
+    while (__ballot(cmask != 0)) {  // rare: rounds over the candidate cells
+      const bool is_cand = cmask != 0;
+      const int m = is_cand ? __builtin_ctz(cmask) : 0;
+      cmask &= cmask - 1u;
+      int32_t q = dv[0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) q = m == k ? dv[k] : q;
+      const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
+      bool emit = false, to_band = false;
+      if (is_cand) {
+        // the cell's error bound e (fixed point) is the smaller of its two
+        // rows' bounds; with rqI.z = ceil(t 2^S) and rqI.w = floor(t 2^S):
+        // q - e >= ceil(t 2^S) makes s >= t certain, q + e < floor(t 2^S)
+        // makes s < t certain, and between them the FP64 rescore decides
+        const int4 bi = rqI[il];
+        const int2 bj = rqJ[jb][jl];
+        int64_t e = bi.x < bj.x ? bi.x : bj.x;
+        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
+        if ((int64_t)q - e >= (int64_t)bi.z) {
+          // certain hit; its score is final only if the bound meets the
+          // relative tolerance (q >= the rows' Q, which is e scaled by it)
+          if (a.rescore_all || q < (bi.y < bj.y ? bi.y : bj.y)) to_band = true; else emit = true;
+        } else if ((int64_t)q + e >= (int64_t)bi.w) {
+          to_band = true;
+        }
+      }
+      // emitted pairs and error-band pairs go to the LDS staging buffers
+      apss_pair pr;
+      pr.i = (uint32_t)(i0 + il + a.i_off);
+      pr.j = (uint32_t)(j0 + jl + a.j_off);
+      pr.score = (float)q * a.qinv_f;  // = (float)(q 2^-S): 2^-S is a power of two
+      stage_append(emit, pr, stage.out, obuf_cap<DM>(), &stage.n_out, a.out_cnt, a.out, a.out_cap);
+      const unsigned long long key = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
+      stage_append(to_band, key, stage.band, bbuf_cap<DM>(), &stage.n_band, a.band_cnt, a.band, a.band_cap);
+    }
+  };
+
+  if constexpr (SP) {
+    // ---- sparse band (SP): tiles with cursor products only -------------------
+    // The host sends here the bands whose rows hold cursor-tier features only
+    // (no bitmap-tier entry, no dense-head or head feature) and whose
+    // thresholds are above zero (q_lo > 0).  Their cells receive cursor
+    // products only, so a tile with no product in the band's bucket holds no
+    // pair that can reach t (the two blocks share no feature) and is skipped
+    // whole.
+    // Per chunk of tiles: bucket the products (walk_chunk), then for each tile
+    // with a non-empty bucket add its products into the score tile and settle
+    // it -- unless a block bound shows no cell can be a candidate: the L1 x
+    // max bound of the two blocks (k_tile_l1max) or, once the products are
+    // in, their sum of magnitudes (every cell's score is at most that sum).
+    // The tile block, the bitmap tier and the dense head are never touched.
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
+    const double qs = (double)a.qscale;
+    const float2 bI = a.I.l1m[I];
+    unsigned long long my_skip = 0, my_bound = 0;
+    for (int32_t J = J_first; J < J_end;) {
+      if (nS > 0) walk_chunk(J, tl);
+      const int32_t ce = nS > 0 ? ch_hi : J_end;
+      for (; J < ce; ++J) {
+        const int32_t pb = nS > 0 ? (int32_t)bkt[J - ch_lo] : 0;
+        const int32_t pe = nS > 0 ? (int32_t)bkt[J - ch_lo + 1] : 0;
+        if (pb == pe) {  // the blocks share no feature
+          ++my_skip;
+          continue;
+        }
+        const float2 bJ = a.J.l1m[J];
+        const double l1m = fmin((double)bI.x * (double)bJ.y, (double)bI.y * (double)bJ.x);
+        if ((l1m * (1.0 + 1e-6) + a.eps_max) * qs < (double)q_lo) {
+          ++my_bound;
+          continue;
+        }
+        const int64_t j0 = a.J.tstart[J];
+        const int nJ = (int)(a.J.tstart[J + 1] - j0);
+        if (tl < TILE_ROWS) rqJ[0][tl] = tl < nJ ? a.J.rq[j0 + tl] : make_int2(0, 0);
+        if (tl == 0) bsum = 0;
+        __syncthreads();
+        // staging buffers three quarters full: empty them (every append of
+        // the earlier tiles is complete behind the barrier above)
+        if (stage.n_out >= obuf_cap<DM>() * 3 / 4 || stage.n_band >= bbuf_cap<DM>() * 3 / 4)
+          flush_stage(a, tl, stage);
+        unsigned long long my_abs = 0;
+        for (int32_t k = pb + tl; k < pe; k += SCAN_THREADS) {
+          const int2 q = cprod[k];
+          atomicAdd(&tile[q.x], q.y);
+          my_abs += (unsigned long long)(q.y < 0 ? -(int64_t)q.y : (int64_t)q.y);
+        }
+        if (my_abs) atomicAdd(&bsum, my_abs);
+        __syncthreads();
+        if (bsum >= (unsigned long long)q_lo) {
+          int32_t dv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          settle(j0, nJ, 0, dv, tl);
+        } else {  // no cell reaches q_lo: clear the cells the products touched
+          for (int32_t k = pb + tl; k < pe; k += SCAN_THREADS) tile[cprod[k].x] = 0;
+          ++my_bound;
+        }
+        __syncthreads();  // cells cleared and appends done before the next tile
+      }
+    }
+    n_skip += my_skip;
+    n_bound += my_bound;
+    __syncthreads();
+    continue;
+  }
+
   Prefetch<DM> pf;
   TileMeta meta_next{};  // metadata of tile J + 1 while tile J is processed
   if (J_first < J_end) {
     prefetch_tile<DM, HM>(a.J, tile_meta(a.J, J_first), J_first, tid, pf);
     meta_next = tile_meta(a.J, J_first + 1 < J_end ? J_first + 1 : J_first);
   }
-  int32_t ch_lo = J_first, ch_hi = nS > 0 ? J_first : J_end;  // bucket chunk [ch_lo, ch_hi)
   // the band's MFMA operands stay in registers for the whole item
   v4i avp[4];
   if constexpr (DM != DENSE_OFF) {
@@ -513,55 +729,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     int tl = tid;
     asm volatile("" : "+v"(tl));
 
-    // ---- 1. cursor tier: bucket the products of the next chunk of tiles ----
-    // Every record walks its row-sorted list up to the chunk end twice: once
-    // to count the products per J tile, once to write them (fixed point, with
-    // their cell) into this workgroup's region, grouped by tile.  Threads take
-    // records one at a time from a shared counter, so long lists spread over
-    // the workgroup.  A chunk whose products exceed the region ends early at a
-    // tile boundary.
-    if (J == ch_hi) {
-      ch_lo = J;
-      const int32_t ce0 = J + CH_TILES < J_end ? J + CH_TILES : J_end;
-      bkt[tl] = 0;
-      if (tl == 0) { bkt[CH_TILES] = 0; ch_end_s = ce0; rec_next = 0; }
-      __syncthreads();
-      unsigned int n_rd = 0;
-      for (;;) {
-        const int k = (int)atomicAdd(&rec_next, 1u);
-        if (k >= nS) break;
-        const int4 st = scs[k];
-        if (st.z < ce0) walk_count(a.S.pos, st.x, st.y, ch_lo, ce0, bkt, n_rd);
-      }
-      if (n_rd) atomicAdd(&acc_walk_c, (unsigned long long)n_rd);
-      __syncthreads();
-      if (tl == 0) rec_next = 0;
-      const uint32_t cnt = bkt[tl];
-      uint32_t total;
-      const uint32_t ex = block_exscan(cnt, tl, wsum, &total);
-      bkt[tl] = ex;
-      bfill[tl] = ex;
-      if (tl == 0) bkt[CH_TILES] = total;
-      if ((int64_t)ex + cnt > a.cprod_cap) atomicMin(&ch_end_s, ch_lo + tl);
-      __syncthreads();
-      const int32_t ce = ch_end_s;
-      for (;;) {
-        const int k = (int)atomicAdd(&rec_next, 1u);
-        if (k >= nS) break;
-        const int4 st = scs[k];
-        if (st.z >= ce) continue;
-        const int4 r = a.I.srec[sb + k];
-        int32_t tn;
-        unsigned int np = 0, nr = 0;
-        const int cur = walk_scatter(a.S.pos, a.S.val, st.x, st.y, ch_lo, ce, r.x,
-                                     __int_as_float(r.y) * a.qscale, bfill, cprod, np, &tn, nr);
-        scs[k] = make_int4(cur, st.y, tn, 0);
-        if (np) atomicAdd(&acc_cursor, (unsigned long long)np);
-        if (nr) atomicAdd(&acc_walk_s, (unsigned long long)nr);
-      }
-      __syncthreads();
-      ch_hi = ce;
-    }
+    if (J == ch_hi) walk_chunk(J, tl);
 
     const int64_t j0 = pf.m.r0;
     const int nJ = pf.m.nrows;
@@ -778,92 +946,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       lds_barrier();
     }
 
-    // ---- 4. threshold and emission ---------------------------------------------
-    // Each thread reads and clears its eight cells (+ their dense parts) and
-    // settles the ones that can reach t itself, one candidate per lane per
-    // round.  Valid cells: row below the band height, column below the tile
-    // height, and for self-joins i0 + il < j0 + jl.  Emitted pairs go to the
-    // LDS output buffer, cells within the error bound to the band list.  No
-    // barrier follows: the next tile pair's first barrier orders the cleared
-    // cells and the buffer appends before their next use.
-    const int64_t dj = j0 - i0;
-    // Cells past the band or tile height receive no products and a zero head,
-    // so with q_lo > 0 only the self-join diagonal needs an explicit validity
-    // test: off the diagonal one max over the eight cells rejects them all.
-    const bool plain = q_lo > 0 && (!a.self || dj >= BAND_ROWS);
-    int32_t qmax = INT32_MIN;
-#pragma unroll
-    for (int blk = 0; blk < 2; ++blk) {  // four band rows of one tile row: one 16-byte word
-      int4* w4 = reinterpret_cast<int4*>(&tile[(jl_b + 16 * blk) * TSTRIDE + il_b]);
-      const int4 c4 = *w4;
-      *w4 = make_int4(0, 0, 0, 0);
-      dv[4 * blk + 0] += c4.x;  // full fixed-point score of the cell
-      dv[4 * blk + 1] += c4.y;
-      dv[4 * blk + 2] += c4.z;
-      dv[4 * blk + 3] += c4.w;
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) qmax = dv[m] > qmax ? dv[m] : qmax;
-    unsigned cmask = 0;
-    if (plain) {
-      // off the self-join diagonal, with q_lo > 0: cells past the band or
-      // tile height hold zero and fail the test by themselves
-      if (__ballot(qmax >= q_lo)) {
-#pragma unroll
-        for (int m = 0; m < 8; ++m) cmask |= dv[m] >= q_lo ? (1u << m) : 0u;
-      }
-    } else {
-      // valid rows of this thread's two blocks: v < lim[blk] (v = m % 4)
-      int lim[2];
-#pragma unroll
-      for (int blk = 0; blk < 2; ++blk) {
-        const int jl = jl_b + 16 * blk;
-        int l = jl < nJ ? nI - il_b : 0;
-        if (a.self) {  // i0 + il < j0 + jl  <=>  il - il_b < dj + jl - il_b
-          const int64_t dl = dj + jl - il_b;
-          l = dl < l ? (int)(dl < 0 ? 0 : dl) : l;
-        }
-        lim[blk] = l < 0 ? 0 : (l > 4 ? 4 : l);
-      }
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        cmask |= ((m & 3) < lim[m >> 2] && dv[m] >= q_lo) ? (1u << m) : 0u;
-    }
-    while (__ballot(cmask != 0)) {  // rare: rounds over the candidate cells
-      const bool is_cand = cmask != 0;
-      const int m = is_cand ? __builtin_ctz(cmask) : 0;
-      cmask &= cmask - 1u;
-      int32_t q = dv[0];
-#pragma unroll
-      for (int k = 1; k < 8; ++k) q = m == k ? dv[k] : q;
-      const int il = il_b + (m & 3), jl = jl_b + 16 * (m >> 2);
-      bool emit = false, to_band = false;
-      if (is_cand) {
-        // the cell's error bound e (fixed point) is the smaller of its two
-        // rows' bounds; with rqI.z = ceil(t 2^S) and rqI.w = floor(t 2^S):
-        // q - e >= ceil(t 2^S) makes s >= t certain, q + e < floor(t 2^S)
-        // makes s < t certain, and between them the FP64 rescore decides
-        const int4 bi = rqI[il];
-        const int2 bj = rqJ[jb][jl];
-        int64_t e = bi.x < bj.x ? bi.x : bj.x;
-        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
-        if ((int64_t)q - e >= (int64_t)bi.z) {
-          // certain hit; its score is final only if the bound meets the
-          // relative tolerance (q >= the rows' Q, which is e scaled by it)
-          if (a.rescore_all || q < (bi.y < bj.y ? bi.y : bj.y)) to_band = true; else emit = true;
-        } else if ((int64_t)q + e >= (int64_t)bi.w) {
-          to_band = true;
-        }
-      }
-      // emitted pairs and error-band pairs go to the LDS staging buffers
-      apss_pair pr;
-      pr.i = (uint32_t)(i0 + il + a.i_off);
-      pr.j = (uint32_t)(j0 + jl + a.j_off);
-      pr.score = (float)q * a.qinv_f;  // = (float)(q 2^-S): 2^-S is a power of two
-      stage_append(emit, pr, stage.out, obuf_cap<DM>(), &stage.n_out, a.out_cnt, a.out, a.out_cap);
-      const unsigned long long key = ((unsigned long long)(i0 + il) << 32) | (unsigned long long)(j0 + jl);
-      stage_append(to_band, key, stage.band, bbuf_cap<DM>(), &stage.n_band, a.band_cnt, a.band, a.band_cap);
-    }
+    // ---- 4. threshold and emission (settle, above) --------------------------
+    settle(j0, nJ, jb, dv, tl);
   }
   __syncthreads();
   }  // band queue
@@ -882,15 +966,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
     atomicAdd(&a.stats[5], acc_band_s);
     atomicAdd(&a.stats[6], acc_walk_c);
     atomicAdd(&a.stats[8], acc_walk_s);
+    atomicAdd(&a.stats[STAT_SKIP_NO_COMMON], n_skip);
+    atomicAdd(&a.stats[STAT_SKIP_BOUND], n_bound);
   }
 }
 
-template __global__ void k_band_scan<DENSE_OFF, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_I8, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_BF16, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_OFF, true>(ScanArgs);
-template __global__ void k_band_scan<DENSE_I8, true>(ScanArgs);
-template __global__ void k_band_scan<DENSE_BF16, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, true, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, true, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, true, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false, true>(ScanArgs);
 
 // FP64 rescore of the band: exact products summed in ascending feature id,
 // the same order as the FP64 oracle, so the decision s >= t is settled exactly.

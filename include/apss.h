@@ -124,6 +124,10 @@ typedef struct {
   int64_t walk_scatter_entries; /* cursor-list entries (position + value) read by the
                               scatter walks */
   int64_t seg_row_bytes;   /* bytes of one J tile's segment-start row, fetched per tile pair */
+  int64_t tile_pairs_skipped; /* tile pairs of sparse bands (rows with rare features only)
+                              skipped: the two blocks share no feature */
+  int64_t tile_pairs_pruned;  /* tile pairs of sparse bands whose block bound (L1 x max of the
+                              blocks, or the sum of their products) stays below t */
   /* timings in milliseconds (hipEvent on the run's stream) */
   double ms_upload, ms_index, ms_scan, ms_rescore, ms_download, ms_total;
   double ms_tune;          /* autotuner (p < 0): time spent timing candidate configurations */

@@ -70,7 +70,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(apss.CSR) == 56
     assert ctypes.sizeof(apss.Opts) == 72
     assert apss.PAIR_DTYPE.itemsize == 12
-    assert ctypes.sizeof(apss.Result) == 240
+    assert ctypes.sizeof(apss.Result) == 256
 
 
 def _nm(path, dynamic):
