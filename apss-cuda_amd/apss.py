@@ -38,7 +38,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", c_int32), ("precision", c_int32), ("sort_output", c_int32),
                 ("rescore_all", c_int32), ("output_on_device", c_int32),
                 ("dense_features", c_int32), ("mem_budget", c_int64), ("stream", c_void_p),
-                ("mfma_head", c_int32), ("band_rows", c_int32), ("reserved_", c_int64 * 3)]
+                ("mfma_head", c_int32), ("band_rows", c_int32), ("count_work", c_int32),
+                ("reserved32_", c_int32), ("reserved_", c_int64 * 2)]
 
 
 class Result(ctypes.Structure):

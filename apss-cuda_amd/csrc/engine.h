@@ -36,7 +36,7 @@ __global__ void k_dense_rows(int64_t n, const int64_t* rp, const int32_t* col, c
                              float* dnorm);
 __global__ void k_cursor_pos(int64_t n, const int2* sent, const int32_t* row2tile,
                              const int64_t* tstart, uint32_t* spos, float* sval);
-template <int DM, bool HM, bool SP>
+template <int DM, bool HM, bool SP, bool ST>
 __global__ void k_band_scan(ScanArgs a);
 __global__ void k_tile_l1max(const int64_t* tstart, const int64_t* rp, const float* val, float2* out);
 __global__ void k_row_bounds(int64_t n, const int64_t* rp, const float* norm, const float* hnorm,
@@ -220,6 +220,8 @@ struct RunConfig {
                                        // estimated full scan time; see tune_bands)
   int32_t tune_bands = 0;              // tune_only: sample this many bands (0: all work items)
   int32_t items_per_cu = 8;            // work items per CU (sets the partner-tile chunk)
+  bool count_work = false;             // count the bitmap-tier list visits and products (a
+                                       // slower form of the band kernel)
   apss_pair* out_given = nullptr;      // output buffer owned by the caller (device): no size
   int64_t out_given_cap = 0;           // estimate and no regrowth; a larger result is
                                        // counted (ScanOutput::count) but not all stored

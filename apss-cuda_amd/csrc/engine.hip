@@ -840,15 +840,20 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
 // ---------------------------------------------------------------------------
 // scan + rescore
 
-// The band kernel's entry for a dense-head mode and split (graph nodes).
-static void* band_scan_fn(int32_t dm, bool hm) {
+// The band kernel's entry for a dense-head mode, split and work counting
+// (graph nodes).
+template <bool ST>
+static void* band_scan_fn_t(int32_t dm, bool hm) {
   if (hm)
-    return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, true, false>)
-         : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, true, false>)
-                            : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, true, false>);
-  return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, false, false>)
-       : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, false, false>)
-                          : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, false>);
+    return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, true, false, ST>)
+         : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, true, false, ST>)
+                            : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, true, false, ST>);
+  return dm == DENSE_I8 ? reinterpret_cast<void*>(&k_band_scan<DENSE_I8, false, false, ST>)
+       : dm == DENSE_BF16 ? reinterpret_cast<void*>(&k_band_scan<DENSE_BF16, false, false, ST>)
+                          : reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, false, ST>);
+}
+static void* band_scan_fn(int32_t dm, bool hm, bool st) {
+  return st ? band_scan_fn_t<true>(dm, hm) : band_scan_fn_t<false>(dm, hm);
 }
 
 namespace {
@@ -893,15 +898,10 @@ JobGraph& job_graph(int device) {
 }  // namespace
 
 // The band kernel for a dense-head mode; HM: the split p > 0 (prefix-norm
-// filter and forward-list head dot) is compiled in.
-template <bool HM>
-static void launch_band_scan(int32_t dm, int g, hipStream_t st, const ScanArgs& a) {
-  if (dm == DENSE_I8)
-    hipLaunchKernelGGL((k_band_scan<DENSE_I8, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
-  else if (dm == DENSE_BF16)
-    hipLaunchKernelGGL((k_band_scan<DENSE_BF16, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_band_scan<DENSE_OFF, HM, false>), dim3(g), dim3(SCAN_THREADS), 0, st, a);
+// filter and forward-list head dot) is compiled in; ST: work counters.
+static void launch_band_scan(int32_t dm, bool hm, bool st, int g, hipStream_t sm, const ScanArgs& a) {
+  void* args[] = {const_cast<ScanArgs*>(&a)};
+  APSS_HIP(hipLaunchKernel(band_scan_fn(dm, hm, st), dim3(g), dim3(SCAN_THREADS), args, 0, sm));
 }
 
 ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const TileSet& bands,
@@ -1141,11 +1141,10 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     split_args(d, sp);
     if (d.n_items > 0) {
       const int gd = std::max(1, std::min(g, d.n_items));
-      if (a.head_mode) launch_band_scan<true>(tr.dm, gd, st, d);
-      else launch_band_scan<false>(tr.dm, gd, st, d);
+      launch_band_scan(tr.dm, a.head_mode != 0, rc.count_work, gd, st, d);
     }
     if (sp.n_items > 0)
-      hipLaunchKernelGGL((k_band_scan<DENSE_OFF, false, true>), dim3(std::max(1, std::min(g, sp.n_items))),
+      hipLaunchKernelGGL((k_band_scan<DENSE_OFF, false, true, false>), dim3(std::max(1, std::min(g, sp.n_items))),
                          dim3(SCAN_THREADS), 0, st, sp);
     APSS_HIP(hipGetLastError());
   };
@@ -1297,11 +1296,11 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       np[0].gridDim = dim3(1);
       np[0].blockDim = dim3(256);
       np[0].kernelParams = fill_args;
-      np[1].func = band_scan_fn(tr.dm, a.head_mode != 0);
+      np[1].func = band_scan_fn(tr.dm, a.head_mode != 0, rc.count_work);
       np[1].gridDim = dim3(std::max(1, std::min(grid, d.n_items)));
       np[1].blockDim = dim3(SCAN_THREADS);
       np[1].kernelParams = scan_args;
-      np[2].func = reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, true>);
+      np[2].func = reinterpret_cast<void*>(&k_band_scan<DENSE_OFF, false, true, false>);
       np[2].gridDim = dim3(std::max(1, std::min(grid, sp.n_items)));
       np[2].blockDim = dim3(SCAN_THREADS);
       np[2].kernelParams = sparse_args;
@@ -1361,10 +1360,11 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       bcap = std::max(bcap, nband + 1024);
       continue;
     }
-    so.products = (int64_t)h[2];
+    // without count_work the bitmap-tier visits and products are not counted
+    so.products = rc.count_work ? (int64_t)h[2] : -1;
     so.head_pairs = (int64_t)h[3];
     so.cells = host_cells;
-    so.visits = (int64_t)h[5];
+    so.visits = rc.count_work ? (int64_t)h[5] : -1;
     so.cursor = (int64_t)h[6];
     so.dense_rows = host_dense;
     so.tile_pairs = partner_tiles;
@@ -1707,6 +1707,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   rc.self = 1;
   rc.rescore_all = o->rescore_all;
   rc.items_per_cu = tc.items_per_cu;
+  rc.count_work = o->count_work != 0;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   out->ms_scan = so.ms_scan;
   out->ms_rescore = so.ms_rescore;
@@ -1920,6 +1921,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   rc.t_min = t_min;
   rc.use_graph = true;  // the batch's device job is one graph launch
   rc.items_per_cu = tc.items_per_cu;
+  rc.count_work = o->count_work != 0;
   ScanOutput so = scan_and_rescore(ar, st, s, bands, s, tiles, tr, rc);
   // split the pair list by instance
   std::vector<apss_pair> h((size_t)so.count);

@@ -91,6 +91,7 @@ int run_noalloc(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss
   rc.out_given = pairs;
   rc.out_given_cap = capacity;
   rc.cprod_floor = (int64_t)1 << 16;
+  rc.count_work = o->count_work != 0;
   ScanOutput so = scan_and_rescore(ar, st, s, bc.bands, s, bc.tiles, bc.tr, rc);
   APSS_HIP(hipStreamSynchronize(st));
   *count = so.count;

@@ -208,6 +208,7 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     rc.row_t = drt;
     rc.t_min = t_min;
     rc.out_cap_hint = query_out_hint(Q.n(), sg->side.n);
+    rc.count_work = ix.opts.count_work != 0;
     ScanOutput so = scan_and_rescore(ar, st, qd, bands, sg->side, sg->tiles, sg->tr, rc);
     apss_pair* at = pairs.grow(so.count);
     if (so.count)
@@ -216,8 +217,9 @@ static void query_index(const Index& ix, const HostCSR& Q, const std::vector<int
     if (stats) {
       stats->cells += so.cells;
       stats->band += so.band;
-      stats->products += so.products;
-      stats->list_visits += so.visits;
+      // (-1: the visits and products were not counted)
+      stats->products = (stats->products < 0 || so.products < 0) ? -1 : stats->products + so.products;
+      stats->list_visits = (stats->list_visits < 0 || so.visits < 0) ? -1 : stats->list_visits + so.visits;
       stats->cursor_products += so.cursor;
       stats->dense_rows += so.dense_rows;
       stats->tile_pairs += so.tile_pairs;

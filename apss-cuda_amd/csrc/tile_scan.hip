@@ -363,7 +363,7 @@ __device__ __forceinline__ void stage_append(bool want, const T& rec, T* buf, un
 
 // Persistent band kernel: one workgroup per CU takes bands from a queue.
 // DM selects the dense head (DENSE_OFF, DENSE_I8, DENSE_BF16).
-template <int DM, bool HM, bool SP>
+template <int DM, bool HM, bool SP, bool ST>
 __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   using Z = Sizes<DM>;
   __shared__ __align__(16) int32_t tile[TILE_ROWS * TSTRIDE];  // fixed-point scores (x 2^S), transposed
@@ -814,7 +814,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
       for (int q = 0; q < Z::IPT / 2; ++q) {
         if (y1[q] <= y0[q]) continue;
         // tile-side list entries: counted once per feature and tile pair
-        my_visits += (unsigned)((e_rf[h + q] >> 26) & 1) * (unsigned)(y1[q] - y0[q]);
+        // (ST: the work counters cost about 2% of the scan, so they are a form
+        // of their own)
+        if constexpr (ST) my_visits += (unsigned)((e_rf[h + q] >> 26) & 1) * (unsigned)(y1[q] - y0[q]);
         const int ib = (e_rf[h + q] >> 12) & 0x3fff;  // band row
         const float u = vI[tl + (h + q) * SCAN_THREADS];
         // four entries per step: their LDS reads issue together, then the atomics
@@ -830,7 +832,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
           const int2 e = eJ[y];
           atomicAdd(&tile[ib + e.x], (int)(u * __int_as_float(e.y)));
         }
-        my_products += (unsigned)(y1[q] - y0[q]);
+        if constexpr (ST) my_products += (unsigned)(y1[q] - y0[q]);
       }
     }
 
@@ -971,13 +973,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
   }
 }
 
-template __global__ void k_band_scan<DENSE_OFF, false, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_I8, false, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_BF16, false, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_OFF, true, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_I8, true, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_BF16, true, false>(ScanArgs);
-template __global__ void k_band_scan<DENSE_OFF, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, false, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, false, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, true, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, true, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, true, false, false>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, false, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, false, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, true, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_I8, true, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_BF16, true, false, true>(ScanArgs);
+template __global__ void k_band_scan<DENSE_OFF, false, true, false>(ScanArgs);
 
 // FP64 rescore of the band: exact products summed in ascending feature id,
 // the same order as the FP64 oracle, so the decision s >= t is settled exactly.

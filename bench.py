@@ -471,7 +471,14 @@ def main():
     last = infos[-1]
     cells = n * (n - 1) / 2.0
 
-    # band-scan roofline: algorithmic bytes / hipEvent kernel time (same stream)
+    # band-scan roofline: algorithmic bytes / hipEvent kernel time (same stream).
+    # The timed steps run the band kernel without its work counters; one more
+    # run of the same job with count_work = 1 (untimed) counts the visits and
+    # products the byte model needs (the work is the same: the pair set and
+    # the tiling do not change).
+    counted = run_device(Xc, args.t, args.p, stream, count_work=1, **opts)
+    assert counted["count"] == last["count"] and counted["tile_pairs"] == last["tile_pairs"]
+    last = dict(last, list_visits=counted["list_visits"], products=counted["products"])
     terms = alg_terms(last)
     ab = sum(terms.values())
     scan_ms = float(np.mean([i["ms_scan"] for i in infos]))

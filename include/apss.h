@@ -97,7 +97,11 @@ typedef struct {
                               -1 = off (every feature on the sparse path) */
   int32_t band_rows;       /* height cap of the bands, 2..128 (0 = 128); tiles take half;
                               p < 0 lets the autotuner choose */
-  int64_t reserved_[3];
+  int32_t count_work;      /* 1: count the bitmap-tier list visits and products
+                              (apss_result.list_visits / products; -1 otherwise) with a
+                              form of the band kernel that is about 2% slower */
+  int32_t reserved32_;
+  int64_t reserved_[2];
 } apss_opts;
 
 typedef struct {
@@ -108,10 +112,11 @@ typedef struct {
   /* counters */
   int64_t cells;           /* pair cells evaluated on the GPU */
   int64_t band;            /* pairs sent to the FP64 rescore */
-  int64_t products;        /* tail partial products accumulated */
+  int64_t products;        /* partial products accumulated (-1 unless opts.count_work) */
   int64_t head_pairs;      /* pairs that ran the head forward-list dot */
   int64_t list_visits;     /* tile-side inverted-list entries visited: the J tile's segment of
-                              every feature the band holds, once per tile pair */
+                              every feature the band holds, once per tile pair (-1 unless
+                              opts.count_work) */
   int64_t cursor_products; /* products of the short-list (cursor) tier */
   int64_t mfma_features;   /* features in the dense MFMA head (0: none) */
   int64_t dense_rows;      /* 256-byte dense head records fetched by the scan */
