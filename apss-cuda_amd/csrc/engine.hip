@@ -614,7 +614,7 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
   // per-tile L1 x max statistics (block bounds of sparse bands)
   ts.l1m = ar.alloc<float2>(std::max(T, 1));
   if (T > 0) {
-    hipLaunchKernelGGL(k_tile_l1max, dim3(T), dim3(128), 0, st, ts.tstart, s.rp, s.val, ts.l1m);
+    hipLaunchKernelGGL(k_tile_l1max, dim3(T), dim3(256), 0, st, ts.tstart, s.rp, s.val, ts.l1m);
     hipLaunchKernelGGL(k_tile_scatter, dim3(T), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_tile_compact, dim3(T), dim3(256), 0, st, b);
     if (records && ts.nS > 0)
@@ -753,6 +753,7 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
     Build* bd = nullptr;
     for (auto& x : builds)
       if (same(x->key, c)) bd = x.get();
+    const auto tb = std::chrono::steady_clock::now();
     if (!bd) {
       auto nb = std::make_unique<Build>();
       nb->key = c;
@@ -763,7 +764,9 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
                            c.band_rows, c.tile_rows);
       bd = nb.get();
       builds.push_back(std::move(nb));
+      if (trace_on()) APSS_HIP(hipStreamSynchronize(st));
     }
+    const auto ts = std::chrono::steady_clock::now();
     RunConfig rc;
     cfg(rc, bd->b);
     rc.p = c.p;
@@ -772,9 +775,15 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
     rc.items_per_cu = c.items_per_cu;
     ScanOutput so = scan_and_rescore(*bd->ar, st, bd->side, bd->b.bands, bd->side, bd->b.tiles, bd->b.tr, rc);
     ++best.candidates;
-    if (trace_on())
-      std::fprintf(stderr, "[apss] tune: p %d dense %d bitmap cap %d rows %d/%d items/CU %d -> %.2f ms\n",
-                   c.p, bd->b.tr.dm, c.kd_cap, c.band_rows, c.tile_rows, c.items_per_cu, so.ms_scan);
+    if (trace_on()) {
+      const auto te = std::chrono::steady_clock::now();
+      std::fprintf(stderr,
+                   "[apss] tune: p %d dense %d bitmap cap %d rows %d/%d items/CU %d -> %.2f ms "
+                   "(build %.1f ms, scan wall %.1f ms)\n",
+                   c.p, bd->b.tr.dm, c.kd_cap, c.band_rows, c.tile_rows, c.items_per_cu, so.ms_scan,
+                   std::chrono::duration<double, std::milli>(ts - tb).count(),
+                   std::chrono::duration<double, std::milli>(te - ts).count());
+    }
     return std::make_pair(so.ms_scan, bd->b.tr.dm);
   };
   const int32_t head = dense_mode(o);
@@ -783,7 +792,9 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
   cur.kd_cap = o.dense_features;
   // 1. the split p (the reference's knob): ids below p take the prefix-norm
   //    filter and the forward-list head dot.  Timed on a sample of 16 bands,
-  //    since a large p can cost a hundred times the scan.
+  //    since a large p can cost a hundred times the scan.  The head-dot work
+  //    only grows with p, so the sweep (ascending p) stops at the first
+  //    candidate slower than twice the best one.
   double best_p_ms = INFINITY;
   for (int32_t p : {0, 1024, 4096}) {
     if (p > s.d) continue;
@@ -793,6 +804,8 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
     if (ms < best_p_ms) {
       best_p_ms = ms;
       cur.p = p;
+    } else if (ms > 2.0 * best_p_ms) {
+      break;
     }
   }
   // 2. the tiering of the tail and the band / tile heights, on the whole prefix

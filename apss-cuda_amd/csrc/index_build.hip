@@ -555,35 +555,40 @@ namespace apss {
 // |s_ij| <= min(max_I |x|_1 * max_J max|y|, max_I max|x| * max_J |y|_1), the
 // block form of the reference's L1 x max bound (filter_pair2,
 // similarity_search.cu:533-558).
-__global__ __launch_bounds__(128) void k_tile_l1max(const int64_t* __restrict__ tstart,
+__global__ __launch_bounds__(256) void k_tile_l1max(const int64_t* __restrict__ tstart,
                                                     const int64_t* __restrict__ rp,
                                                     const float* __restrict__ val, float2* out) {
-  __shared__ float sl[128], sm[128];
+  // one wave per row (rows r0 + w, r0 + w + 4, ...), lanes over its entries:
+  // the value loads of a wave are contiguous
+  __shared__ float sl[4], sm[4];
   const int t = blockIdx.x;
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int64_t r0 = tstart[t], r1 = tstart[t + 1];
   float l1 = 0.f, mx = 0.f;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+  for (int64_t r = r0 + w; r < r1; r += 4) {
     double sum = 0.0;
     float m = 0.f;
-    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+    for (int64_t k = rp[r] + ln; k < rp[r + 1]; k += 64) {
       const float v = fabsf(val[k]);
       sum += (double)v;
       m = fmaxf(m, v);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sum += __shfl_xor(sum, o);
+      m = fmaxf(m, __shfl_xor(m, o));
+    }
     l1 = fmaxf(l1, (float)sum * (1.0f + 1e-6f));  // covers the rounding to FP32
     mx = fmaxf(mx, m);
   }
-  sl[threadIdx.x] = l1;
-  sm[threadIdx.x] = mx;
-  __syncthreads();
-  for (int o = 64; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      sl[threadIdx.x] = fmaxf(sl[threadIdx.x], sl[threadIdx.x + o]);
-      sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + o]);
-    }
-    __syncthreads();
+  if (ln == 0) {
+    sl[w] = l1;
+    sm[w] = mx;
   }
-  if (threadIdx.x == 0) out[t] = make_float2(sl[0], sm[0]);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    out[t] = make_float2(fmaxf(fmaxf(sl[0], sl[1]), fmaxf(sl[2], sl[3])),
+                         fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3])));
 }
 
 }  // namespace apss
