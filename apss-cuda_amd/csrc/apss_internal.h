@@ -80,6 +80,7 @@ enum : uint32_t {
   VERR_ORDER = 4u,    // columns not strictly ascending in a row
   VERR_VALUE = 8u,    // non-finite value
   VERR_NORM = 16u,    // row norm too large for the FP32 path
+  VFLAG_NEG = 1u << 31,  // not an error: the matrix holds a negative value
 };
 
 // ---------------------------------------------------------------------------
@@ -167,6 +168,9 @@ struct ScanArgs {
   float qscale;              // 2^S
   float qinv_f;              // 2^-S (exact in FP32)
   float cbf_q;               // BF16 head: c_bf 2^S rounded up (per-cell part of the bound)
+  uint32_t cabs32;           // BF16 head: c_bf / (1 - c_bf) x 2^32 rounded up
+  int32_t bf_nonneg;         // BF16 head: no negative value on either side (the head
+                             // error is bounded from the cell's own score)
   int32_t q_lo;              // cells below this cannot reach t (fast reject)
   double qinv;               // 2^-S
   double score_rel;          // emitted scores must be within this relative error

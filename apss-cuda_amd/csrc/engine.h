@@ -42,10 +42,10 @@ __global__ void k_tile_l1max(const int64_t* tstart, const int64_t* rp, const flo
 __global__ void k_row_bounds(int64_t n, const int64_t* rp, const float* norm, const float* hnorm,
                              RowBoundArgs b, int2* out);
 __global__ void k_rescore(const unsigned long long* band, const unsigned long long* band_cnt,
-                          int64_t band_cap, const int64_t* rpI,
-                          const int32_t* colI, const float* valI, const int64_t* rpJ,
-                          const int32_t* colJ, const float* valJ, double t, const double* row_t,
-                          int64_t i_off, int64_t j_off, apss_pair* out,
+                          int64_t band_cap, const int64_t* rpI, const int32_t* colI,
+                          const float* valI, int64_t nnzI, const int64_t* rpJ,
+                          const int32_t* colJ, const float* valJ, int64_t nnzJ, double t,
+                          const double* row_t, int64_t i_off, int64_t j_off, apss_pair* out,
                           unsigned long long* out_cnt, int64_t out_cap);
 
 // ---- device memory --------------------------------------------------------
@@ -157,6 +157,7 @@ struct Side {
   float* hnorm = nullptr;
   int32_t maxlen = 0;
   float maxnorm = 0.f;
+  bool nonneg = false;           // no negative value (set by the validation)
   // dense head records (n x 256 bytes) for one tiering of the corpus
   uint4* dense = nullptr;
   float* dnorm = nullptr;        // BF16 head: norm of each row's head part

@@ -318,7 +318,8 @@ Side prepare_side(Arena& ar, hipStream_t st, const apss_csr& X, int32_t p) {
   APSS_HIP(hipMemcpyAsync(herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
   APSS_HIP(hipStreamSynchronize(st));
   ar.release(err);
-  if (herr[0]) throw Error{APSS_E_INVALID, "invalid CSR:" + verr_text(herr[0])};
+  if (herr[0] & ~VFLAG_NEG) throw Error{APSS_E_INVALID, "invalid CSR:" + verr_text(herr[0])};
+  s.nonneg = (herr[0] & VFLAG_NEG) == 0;
   // maxima for the global fast-reject bound
   s.maxlen = (int32_t)herr[1];
   std::memcpy(&s.maxnorm, &herr[2], sizeof(float));
@@ -1114,6 +1115,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
   a.qscale = (float)qscale;
   a.qinv_f = (float)qinv;
   a.cbf_q = std::nextafter((float)(c_bf * qscale), INFINITY);
+  a.cabs32 = (uint32_t)std::ceil(std::ldexp(c_bf / (1.0 - c_bf), 32) * (1.0 + 1e-12));
+  a.bf_nonneg = (I.nonneg && J.nonneg) ? 1 : 0;
   a.qinv = qinv;
   a.q_lo = q_lo;
   a.score_rel = 1e-6;  // (the kernel applies it through the rows' Q bounds)
@@ -1264,8 +1267,8 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
     launch(grid);
     if (timed) APSS_HIP(hipEventRecord(ev[1], st));
     hipLaunchKernelGGL(k_rescore, dim3(rs_grid), dim3(256), 0, st, band, ctr + 1, bc, I.rp, I.col,
-                       I.val, J.rp, J.col, J.val, rc.t, rc.row_t, rc.i_off, rc.j_off, out, ctr + 0,
-                       ocap);
+                       I.val, I.nnz, J.rp, J.col, J.val, J.nnz, rc.t, rc.row_t, rc.i_off, rc.j_off,
+                       out, ctr + 0, ocap);
     APSS_HIP(hipGetLastError());
     if (timed) APSS_HIP(hipEventRecord(ev[2], st));
   };
@@ -1292,17 +1295,20 @@ ScanOutput scan_and_rescore(Arena& ar, hipStream_t st, const Side& I, const Tile
       const int64_t* r_rpI = I.rp;
       const int32_t* r_colI = I.col;
       const float* r_valI = I.val;
+      int64_t r_nnzI = I.nnz;
       const int64_t* r_rpJ = J.rp;
       const int32_t* r_colJ = J.col;
       const float* r_valJ = J.val;
+      int64_t r_nnzJ = J.nnz;
       double r_t = rc.t;
       const double* r_rowt = rc.row_t;
       int64_t r_ioff = rc.i_off, r_joff = rc.j_off;
       apss_pair* r_out = out;
       unsigned long long* r_ocnt = ctr + 0;
       int64_t r_ocap = cap;
-      void* rs_args[] = {&r_band, &r_cnt, &r_bcap, &r_rpI, &r_colI, &r_valI, &r_rpJ, &r_colJ,
-                         &r_valJ, &r_t, &r_rowt, &r_ioff, &r_joff, &r_out, &r_ocnt, &r_ocap};
+      void* rs_args[] = {&r_band, &r_cnt,  &r_bcap, &r_rpI, &r_colI, &r_valI,
+                         &r_nnzI, &r_rpJ,  &r_colJ, &r_valJ, &r_nnzJ, &r_t,
+                         &r_rowt, &r_ioff, &r_joff, &r_out,  &r_ocnt, &r_ocap};
       // (a scan node with no items exits at once)
       hipKernelNodeParams np[4] = {};
       np[0].func = reinterpret_cast<void*>(&k_fill_u64);

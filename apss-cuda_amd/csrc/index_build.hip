@@ -44,8 +44,9 @@ __global__ void k_copy_i64(int64_t* dst, const int64_t* src, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// validation + norms: one wave per row.  err[0] collects the error bits,
-// err[1] and err[2] (zeroed by the caller) the longest row and the largest norm.
+// validation + norms: one wave per row.  err[0] collects the error bits (and
+// VFLAG_NEG, which is no error), err[1] and err[2] (zeroed by the caller) the
+// longest row and the largest norm.
 
 __device__ __forceinline__ float round_up_f(double s) {
   float f = (float)s;
@@ -80,6 +81,7 @@ __global__ __launch_bounds__(256) void k_validate_norms(int64_t n, int64_t d, in
       if (c < 0 || c >= d) e_bits |= VERR_COL;
       if (e + 1 < b && col[e + 1] <= c) e_bits |= VERR_ORDER;
       if (!isfinite(v)) e_bits |= VERR_VALUE;
+      if (v < 0.f) e_bits |= VFLAG_NEG;
       double v2 = (double)v * (double)v;
       ss += v2;
       if (c < p) hs += v2;

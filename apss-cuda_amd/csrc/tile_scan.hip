@@ -164,6 +164,20 @@ __device__ __forceinline__ int32_t bf_cell(float cbf_q, float dn_i, float dn_j) 
   return (int32_t)ceilf(cbf_q * dn_i * dn_j * (1.0f + 0x1p-20f)) + 1;
 }
 
+// BF16 head with no negative value on either side: every product of a cell
+// is non-negative, so the head part A of its score is at most the whole score
+// s.  With q the cell's fixed-point score and e its other error terms,
+// s <= (q + e) 2^-S + c_bf A gives A <= (q + e) 2^-S / (1 - c_bf), so the head
+// error c_bf A is at most c_bf / (1 - c_bf) (q + e) fixed-point units
+// (cabs32 = that factor x 2^32 rounded up; +2 covers the floor of the high
+// product and a margin).  This replaces the norm bound c_bf |x_H||y_H|, which
+// is wider for the cells near t.
+__device__ __forceinline__ int32_t bf_abs(uint32_t cabs32, int32_t q, int64_t e) {
+  const int64_t qe = (int64_t)q + e;
+  const uint32_t u = qe <= 0 ? 0u : qe >= (int64_t)0xffffffffLL ? 0xffffffffu : (uint32_t)qe;
+  return (int32_t)__umulhi(u, cabs32) + 2;
+}
+
 // The J side of the kernel's arguments, copied from the kernarg segment where
 // it is used: the opaque pointer keeps the compiler from hoisting the loads
 // out of the tile loop, so these pointers hold no scalar registers between
@@ -620,7 +634,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         const int4 bi = rqI[il];
         const int2 bj = rqJ[jb][jl];
         int64_t e = bi.x < bj.x ? bi.x : bj.x;
-        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
+        if constexpr (DM == DENSE_BF16)
+          e += a.bf_nonneg ? bf_abs(a.cabs32, q, e) : bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
         if ((int64_t)q - e >= (int64_t)bi.z) {
           // certain hit; its score is final only if the bound meets the
           // relative tolerance (q >= the rows' Q, which is e scaled by it)
@@ -934,7 +949,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_band_scan(ScanArgs a) {
         // 2^-S <= t, so the test only prunes less)
         const int4 bi = rqI[il];
         int64_t e = bi.x < rqJ[jb][jl].x ? bi.x : rqJ[jb][jl].x;
-        if constexpr (DM == DENSE_BF16) e += bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
+        if constexpr (DM == DENSE_BF16)
+          e += a.bf_nonneg ? bf_abs(a.cabs32, part, e) : bf_cell(a.cbf_q, dnI[il], dnJ[jb][jl]);
         const double bound = (double)((int64_t)part + e) * a.qinv + (double)hnI[il] * (double)hnJ[jb][jl];
         if (bound < (double)bi.w * a.qinv) continue;
         const int64_t ri = i0 + il, rj = j0 + jl;
@@ -992,21 +1008,86 @@ template __global__ void k_band_scan<DENSE_OFF, false, true, false>(ScanArgs);
 // Grid-stride over the band count the scan left in device memory (capped at
 // the list's capacity), so the scan and the rescore follow each other on the
 // stream with no host round trip and can be captured in one graph.
+//
+// Each lane merges its pair's two sorted rows through windows of four column
+// ids and four values, read as aligned 16-byte words (the window holding the
+// merge position, and the next one, loaded a window ahead) and advanced every
+// fourth step of a row: the steps between, matches included, touch registers
+// only.  Windows at an array's end, or of arrays that are not 16-byte aligned,
+// are read entry by entry.
+constexpr unsigned int RS_STAGE = 1024;  // emitted pairs staged per workgroup of the rescore
+struct RowWindow {
+  int4 c;     // column ids of the aligned word holding the merge position
+  float4 v;   // their values
+};
+// the aligned 4-entry word at col + b (b a multiple of 4); `room` = entries
+// of the array from b on
+__device__ __forceinline__ RowWindow row_window(const int32_t* __restrict__ col,
+                                                const float* __restrict__ val, int64_t b,
+                                                int64_t room, bool vec) {
+  RowWindow w;
+  if (vec && room >= 4) {
+    w.c = *reinterpret_cast<const int4*>(col + b);
+    w.v = *reinterpret_cast<const float4*>(val + b);
+  } else {
+    w.c.x = room > 0 ? col[b] : INT32_MAX;
+    w.c.y = room > 1 ? col[b + 1] : INT32_MAX;
+    w.c.z = room > 2 ? col[b + 2] : INT32_MAX;
+    w.c.w = room > 3 ? col[b + 3] : INT32_MAX;
+    w.v.x = room > 0 ? val[b] : 0.f;
+    w.v.y = room > 1 ? val[b + 1] : 0.f;
+    w.v.z = room > 2 ? val[b + 2] : 0.f;
+    w.v.w = room > 3 ? val[b + 3] : 0.f;
+  }
+  return w;
+}
+// entry k of a window: two levels of selects (no branches)
+__device__ __forceinline__ int32_t pick_i(const int4& w, int k) {
+  const int32_t lo = (k & 1) ? w.y : w.x, hi = (k & 1) ? w.w : w.z;
+  return (k & 2) ? hi : lo;
+}
+__device__ __forceinline__ float pick_f(const float4& w, int k) {
+  const float lo = (k & 1) ? w.y : w.x, hi = (k & 1) ? w.w : w.z;
+  return (k & 2) ? hi : lo;
+}
+
 __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __restrict__ band,
                                                  const unsigned long long* __restrict__ band_cnt,
                                                  int64_t band_cap, const int64_t* __restrict__ rpI,
                                                  const int32_t* __restrict__ colI,
-                                                 const float* __restrict__ valI,
+                                                 const float* __restrict__ valI, int64_t nnzI,
                                                  const int64_t* __restrict__ rpJ,
                                                  const int32_t* __restrict__ colJ,
-                                                 const float* __restrict__ valJ, double t,
-                                                 const double* __restrict__ row_t,
+                                                 const float* __restrict__ valJ, int64_t nnzJ,
+                                                 double t, const double* __restrict__ row_t,
                                                  int64_t i_off, int64_t j_off, apss_pair* out,
                                                  unsigned long long* out_cnt, int64_t out_cap) {
   const int64_t c = (int64_t)*band_cnt;
   const int64_t nband = c < band_cap ? c : band_cap;
-  // the loop bound is uniform per workgroup: every lane takes part in the
-  // wave-wide reservation below
+  const bool vecI = ((reinterpret_cast<uintptr_t>(colI) | reinterpret_cast<uintptr_t>(valI)) & 15) == 0;
+  const bool vecJ = ((reinterpret_cast<uintptr_t>(colJ) | reinterpret_cast<uintptr_t>(valJ)) & 15) == 0;
+  // emitted pairs are staged in LDS and leave with one global reservation per
+  // flush (a per-wave reservation on the one output counter serialised the
+  // kernel); the loop bound is uniform per workgroup, so every thread reaches
+  // the barriers below
+  __shared__ apss_pair buf[RS_STAGE];
+  __shared__ unsigned int n_buf, wcnt[4];
+  __shared__ unsigned long long gbase;
+  const int w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) n_buf = 0;
+  __syncthreads();
+  auto flush = [&]() {
+    if (threadIdx.x == 0) gbase = n_buf ? atomicAdd(out_cnt, (unsigned long long)n_buf) : 0ull;
+    __syncthreads();
+    const unsigned int nb_ = n_buf;
+    for (unsigned int q = threadIdx.x; q < nb_; q += blockDim.x) {
+      const unsigned long long slot = gbase + q;
+      if (slot < (unsigned long long)out_cap) out[slot] = buf[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) n_buf = 0;
+    __syncthreads();
+  };
   for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nband;
        base += (int64_t)gridDim.x * blockDim.x) {
   const int64_t k = base + threadIdx.x;
@@ -1017,25 +1098,62 @@ __global__ __launch_bounds__(256) void k_rescore(const unsigned long long* __res
     unsigned long long pr = band[k];
     i = (uint32_t)(pr >> 32);
     j = (uint32_t)(pr & 0xffffffffull);
-    int64_t x = rpI[i], xe = rpI[i + 1], y = rpJ[j], ye = rpJ[j + 1];
-    while (x < xe && y < ye) {
-      int32_t cx = colI[x], cy = colJ[y];
-      if (cx < cy) ++x;
-      else if (cx > cy) ++y;
-      else { s += (double)valI[x] * (double)valJ[y]; ++x; ++y; }
+    const int64_t x = rpI[i], y = rpJ[j];
+    int na = (int)(rpI[i + 1] - x), nb = (int)(rpJ[j + 1] - y);  // entries left in each row
+    if (na > 0 && nb > 0) {
+      int64_t ba = x & ~(int64_t)3, bb = y & ~(int64_t)3;  // aligned window starts
+      int ka = (int)(x & 3), kb = (int)(y & 3);              // merge positions in the windows
+      // the current window of each row and the next one, loaded a window ahead
+      RowWindow wa = row_window(colI, valI, ba, nnzI - ba, vecI);
+      RowWindow wb = row_window(colJ, valJ, bb, nnzJ - bb, vecJ);
+      RowWindow na_w = row_window(colI, valI, ba + 4, nnzI - ba - 4, vecI);
+      RowWindow nb_w = row_window(colJ, valJ, bb + 4, nnzJ - bb - 4, vecJ);
+      while (true) {
+        const int32_t cx = pick_i(wa.c, ka), cy = pick_i(wb.c, kb);
+        if (cx == cy) s += (double)pick_f(wa.v, ka) * (double)pick_f(wb.v, kb);
+        if (cx <= cy) {
+          if (--na == 0) break;
+          if (++ka == 4) {
+            ka = 0;
+            ba += 4;
+            wa = na_w;
+            na_w = row_window(colI, valI, ba + 4, nnzI - ba - 4, vecI);
+          }
+        }
+        if (cy <= cx) {
+          if (--nb == 0) break;
+          if (++kb == 4) {
+            kb = 0;
+            bb += 4;
+            wb = nb_w;
+            nb_w = row_window(colJ, valJ, bb + 4, nnzJ - bb - 4, vecJ);
+          }
+        }
+      }
     }
     emit = s >= (row_t ? row_t[i] : t);
   }
-  int tot;
-  long long slot = wave_reserve(out_cnt, emit, &tot);
-  if (emit && slot < out_cap) {
+  // this block's emitted pairs of the round, appended behind the buffer's
+  // contents (a round adds at most blockDim.x <= 256 pairs; the buffer is
+  // flushed before a round could overflow it)
+  const unsigned long long m = __ballot(emit);
+  if (lane_id() == 0) wcnt[w] = (unsigned int)__popcll(m);
+  __syncthreads();
+  unsigned int pos = n_buf + (unsigned int)lane_rank(m);
+  for (int v = 0; v < w; ++v) pos += wcnt[v];
+  if (emit) {
     apss_pair p;
     p.i = (uint32_t)(i + i_off);
     p.j = (uint32_t)(j + j_off);
     p.score = (float)s;
-    out[slot] = p;
+    buf[pos] = p;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) n_buf += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+  __syncthreads();
+  if (n_buf > RS_STAGE - 256) flush();
   }
+  if (n_buf > 0) flush();
 }
 
 // Settle bounds of every row of one matrix side for one launch.  E bounds the
