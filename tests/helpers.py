@@ -34,6 +34,13 @@ def check_pairs(got, exp, rel=1e-6):
     return float(err.max())
 
 
+# Emitted scores of the BF16 head on unit rows: the head error is at most
+# c (|x_H||y_H|) <= c with c = 2^-7 + 2^-16 + 2 gamma_128 (7.87e-3), or, with no
+# negative value, c / (1 - c) s <= 7.93e-3 for s <= 1, plus the tail's FP32
+# and fixed-point terms (below 1e-5).
+BF16_ATOL = 8.0e-3
+
+
 def check_pairs_abs(got, exp, atol):
     """Exact pair-set equality and scores within `atol` absolute (BF16 head)."""
     g = apss.sort_pairs(got)

@@ -102,7 +102,7 @@ def test_large_t05_pair_set(large, form):
     count, csum, keys, scores, info = stream(large, 0.5, **opts)
     print(f"{form}: {count} pairs, scan {info['ms_scan']:.0f} ms, rescored {info['band']}", flush=True)
     if form == "bf16_head":
-        # BF16 head bound for unit rows (7.9e-3) + the FP32 tail bound
+        # BF16 head bound for unit rows (at most 7.93e-3) + the FP32 tail bound
         check_sample(large, 0.5, keys, scores, atol=8.1e-3)
     else:
         check_sample(large, 0.5, keys, scores, rel=1e-6)

@@ -9,7 +9,7 @@ import pytest
 
 import apss
 import oracle
-from helpers import check_pairs, check_pairs_abs, subset, tiny_matrix
+from helpers import BF16_ATOL, check_pairs, check_pairs_abs, subset, tiny_matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -204,7 +204,7 @@ def test_bf16_head_pair_set_exact(t):
     rp, col, val, d = apss.workload("mixed", n=6000)
     got, info = apss.run(rp, col, val, d, t, 0, precision=apss.APSS_PREC_BF16_HEAD)
     exp = oracle.apss_fp64(rp, col, val, d, t)
-    err = check_pairs_abs(got, exp, 7.9e-3)
+    err = check_pairs_abs(got, exp, BF16_ATOL)
     assert info["mfma_features"] == 128
     assert info["band"] > 0  # the BF16 bound sends near-threshold cells to the FP64 rescore
     assert err > 0.0
@@ -214,9 +214,39 @@ def test_bf16_head_with_split_and_rescore_all():
     rp, col, val, d = apss.workload("large", n=3000)
     exp = oracle.apss_fp64(rp, col, val, d, 0.45)
     got, _ = apss.run(rp, col, val, d, 0.45, 32, precision=apss.APSS_PREC_BF16_HEAD)
-    check_pairs_abs(got, exp, 7.9e-3)
+    check_pairs_abs(got, exp, BF16_ATOL)
     got, _ = apss.run(rp, col, val, d, 0.45, 0, precision=apss.APSS_PREC_BF16_HEAD, rescore_all=1)
     check_pairs(got, exp)  # every score from the FP64 rescore
+
+
+def test_bf16_head_signed_unit_rows():
+    # negative values: the BF16 head's error is bounded by the head norms (the
+    # bound from the cell's own score needs non-negative products)
+    rp, col, val, d = apss.workload("mixed", n=5000)
+    rng = np.random.default_rng(11)
+    v2 = np.where(rng.random(len(val)) < 0.3, -val, val).astype(np.float32)
+    for t in (0.3, 0.6):
+        got, info = apss.run(rp, col, v2, d, t, 0, precision=apss.APSS_PREC_BF16_HEAD)
+        check_pairs_abs(got, oracle.apss_fp64(rp, col, v2, d, t), BF16_ATOL)
+        assert info["mfma_features"] == 128
+
+
+def test_bf16_head_bound_switches_on_a_negative_value():
+    # the same matrix with and without one more row holding a single negative
+    # value (it pairs with nothing): both pair sets are exact, and without a
+    # negative value the bound from the cell's own score sends fewer cells to
+    # the FP64 rescore than the head-norm bound
+    rp, col, val, d = apss.workload("mixed", n=6000)
+    exp = oracle.apss_fp64(rp, col, val, d, 0.5)
+    got, info_pos = apss.run(rp, col, val, d, 0.5, 0, precision=apss.APSS_PREC_BF16_HEAD)
+    check_pairs_abs(got, exp, BF16_ATOL)
+    rp2 = np.append(rp, rp[-1] + 1).astype(np.int64)
+    col2 = np.append(col, 0).astype(np.int32)
+    val2 = np.append(val, -1.0).astype(np.float32)
+    got2, info_neg = apss.run(rp2, col2, val2, d, 0.5, 0, precision=apss.APSS_PREC_BF16_HEAD)
+    check_pairs_abs(got2, exp, BF16_ATOL)
+    print(f"rescored: {info_pos['band']} without, {info_neg['band']} with a negative value", flush=True)
+    assert 0 < info_pos["band"] < info_neg["band"]
 
 
 def test_text_file_to_pair_file(tmp_path):
