@@ -24,9 +24,11 @@
  *     within its rigorous rounding bound of t is rescored in FP64 on the GPU;
  *   - APSS_PREC_FP32: every emitted score is within 1e-6 relative of the FP64
  *     score.  APSS_PREC_BF16_HEAD: the pair set is still exact; an emitted
- *     score is within (2^-7 + 2^-16 + 2*gamma_128) * |x_H| |y_H| + 2^-S +
- *     (FP32 tail bound) of the FP64 score, |x_H| the norm of the row's
- *     features in the BF16 head (at most 7.9e-3 for unit rows);
+ *     score is within c * |x_H| |y_H| + 2^-S + (FP32 tail bound) of the FP64
+ *     score, c = 2^-7 + 2^-16 + 2*gamma_128 and |x_H| the norm of the row's
+ *     features in the BF16 head, or, when no value of either matrix is
+ *     negative, within c / (1 - c) * s (+ the same terms) for the score s
+ *     (either way at most 7.93e-3 for unit rows);
  *   - errors are returned as negative APSS_E_* codes; nothing calls exit().
  */
 #ifndef APSS_H
