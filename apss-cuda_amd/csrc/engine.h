@@ -239,6 +239,7 @@ struct TuneChoice {
   int32_t candidates = 0;  // configurations timed
   double est_ms = 0.0;     // estimated scan time of the choice
   double ms = 0.0;         // time spent tuning
+  bool cached = false;     // taken from an earlier job of the same shape
 };
 
 // Everything a configuration builds on top of a prepared side.
@@ -258,9 +259,14 @@ Side with_split(Arena& ar, hipStream_t st, const Side& s, int32_t p);
 // configurations in three stages (the split p on a band sample; the tiering --
 // dense head on / off, bitmap tier at its maximum / 1024 features -- and the
 // band / tile heights; the work items per CU) and returns the fastest.  `cfg`
-// fills the run configuration of a candidate's tiling.
+// fills the run configuration of a candidate's tiling; `t` is the threshold
+// the candidates decide at.  A choice is kept for the life of the process per
+// job shape (device, rows, nonzeros, columns, longest row, largest norm,
+// instance starts, threshold, head options): a later job of the same shape
+// takes it without timing (TuneChoice::cached).  The choice changes the
+// speed only, never the pair set.
 TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
-                    const std::vector<int64_t>& bounds,
+                    const std::vector<int64_t>& bounds, double t,
                     const std::function<void(RunConfig&, const Built&)>& cfg);
 
 struct ScanOutput {

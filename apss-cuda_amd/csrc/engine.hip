@@ -18,6 +18,7 @@
 #include <numeric>
 #include <sys/mman.h>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 
 #include "engine.h"
@@ -705,10 +706,56 @@ Side with_split(Arena& ar, hipStream_t st, const Side& s, int32_t p) {
   return r;
 }
 
+namespace {
+// The autotuner's choices by job shape (see autotune).
+struct TuneKey {
+  int device;
+  int64_t n, nnz, d;
+  int32_t maxlen;
+  uint32_t maxnorm_bits;
+  int32_t self;
+  uint64_t bounds_hash;
+  uint64_t t_bits;
+  int32_t dense_features, head;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(device, n, nnz, d, maxlen, maxnorm_bits, self, bounds_hash, t_bits,
+                    dense_features, head) < std::tie(o.device, o.n, o.nnz, o.d, o.maxlen,
+                                                     o.maxnorm_bits, o.self, o.bounds_hash,
+                                                     o.t_bits, o.dense_features, o.head);
+  }
+};
+struct TuneCache {
+  std::mutex mu;
+  std::map<TuneKey, TuneChoice> map;
+};
+TuneCache& tune_cache() {
+  static TuneCache c;
+  return c;
+}
+constexpr size_t TUNE_CACHE_MAX = 256;
+}  // namespace
+
 TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
-                    const std::vector<int64_t>& bounds,
+                    const std::vector<int64_t>& bounds, double t,
                     const std::function<void(RunConfig&, const Built&)>& cfg) {
   const auto t0 = std::chrono::steady_clock::now();
+  TuneKey key{o.device, s.n, s.nnz, s.d, s.maxlen, 0u, self, 1469598103934665603ull, 0u,
+              o.dense_features, dense_mode(o)};
+  std::memcpy(&key.maxnorm_bits, &s.maxnorm, sizeof(float));
+  std::memcpy(&key.t_bits, &t, sizeof(double));
+  for (int64_t b : bounds) key.bounds_hash = (key.bounds_hash ^ (uint64_t)b) * 1099511628211ull;
+  {
+    TuneCache& tcache = tune_cache();
+    std::lock_guard<std::mutex> lk(tcache.mu);
+    const auto it = tcache.map.find(key);
+    if (it != tcache.map.end()) {
+      TuneChoice c = it->second;
+      c.cached = true;
+      c.candidates = 0;
+      c.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      return c;
+    }
+  }
   // Candidates are timed on a prefix of the rows (1/16, at least 16384 and at
   // most 65536 rows): the tiers keep their relative shape (the bitmap-tier
   // threshold scales with the tile count) and each candidate's build costs a
@@ -848,6 +895,12 @@ TuneChoice autotune(hipStream_t st, const apss_opts& o, Side& s, int32_t self,
   }
   APSS_HIP(hipStreamSynchronize(st));
   best.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  {
+    TuneCache& tcache = tune_cache();
+    std::lock_guard<std::mutex> lk(tcache.mu);
+    if (tcache.map.size() >= TUNE_CACHE_MAX) tcache.map.clear();
+    tcache.map[key] = best;
+  }
   return best;
 }
 
@@ -1701,7 +1754,7 @@ int run_self(const apss_csr* X, double t, int32_t p, const apss_opts* o, apss_re
   int32_t dm = dense_mode(*o), kd_cap = o->dense_features;
   TuneChoice tc = choice_from_opts(*o);  // when not tuning
   if (tune) {
-    tc = autotune(st, *o, s, 1, {}, [&](RunConfig& r, const Built&) {
+    tc = autotune(st, *o, s, 1, {}, t, [&](RunConfig& r, const Built&) {
       r.t = t;
       r.self = 1;
     });
@@ -1898,7 +1951,7 @@ int run_batched(const apss_csr* const* X, int32_t count, const double* t, int32_
   double ms_tune = 0.0;
   TuneChoice tc = choice_from_opts(*o);  // when not tuning
   if (tune) {  // the tuner's candidates decide at the lowest threshold of the batch
-    tc = autotune(st, *o, s, 1, bounds, [&](RunConfig& r, const Built& b) {
+    tc = autotune(st, *o, s, 1, bounds, t_min, [&](RunConfig& r, const Built& b) {
       r.t = t_min;
       r.self = 1;
       band_ranges(b.bands, b.tiles, r.band_jend, nullptr);

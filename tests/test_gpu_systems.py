@@ -102,6 +102,24 @@ def test_tuned_shapes_match_oracle(p, br):
     assert (info["band_rows"], info["tile_rows"]) == (br, br // 2)
 
 
+def test_autotuner_choice_kept_per_job_shape():
+    # a second tuned run of the same job shape takes the kept choice without
+    # timing candidates: same configuration, same pairs; another threshold is
+    # another shape and is tuned again
+    rp, col, val, d = small_rows(40_001, seed_offset=131, name="large")
+    a, ia = apss.run(rp, col, val, d, 0.6125, -1)
+    b, ib = apss.run(rp, col, val, d, 0.6125, -1)
+    print(f"tune {ia['ms_tune']:.2f} ms, then {ib['ms_tune']:.3f} ms", flush=True)
+    assert ia["ms_tune"] > 2.0 and ib["ms_tune"] < 1.0
+    for f in ("p_used", "band_rows", "tile_rows", "items_per_cu", "mfma_features", "bitmap_features"):
+        assert ia[f] == ib[f], f
+    ga, gb = apss.sort_pairs(a), apss.sort_pairs(b)
+    assert np.array_equal(ga["i"], gb["i"]) and np.array_equal(ga["j"], gb["j"])
+    check_pairs(b, oracle.apss_fp64(rp, col, val, d, 0.6125))
+    _, ic = apss.run(rp, col, val, d, 0.6375, -1)
+    assert ic["ms_tune"] > 2.0
+
+
 def test_autotuned_batch_prefix_inside_an_instance():
     # the autotuner times candidates on a row prefix; with two 40,000-row
     # instances the prefix (16,384 rows) ends inside the first instance, whose
