@@ -335,6 +335,25 @@ int32_t dense_mode(const apss_opts& o) {
   return o.precision == APSS_PREC_BF16_HEAD ? DENSE_BF16 : DENSE_I8;
 }
 
+// A host array for the index build's transfers, from the pinned block pool
+// (pageable below the pool's minimum block): copies to and from it run at DMA
+// speed, and reused blocks are already faulted in.  Every asynchronous copy
+// that reads or writes it must have completed before it goes out of scope.
+template <class T>
+struct HostBuf {
+  T* p = nullptr;
+  explicit HostBuf(int64_t n) {
+    const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    p = reinterpret_cast<T*>(host_pairs_alloc((int64_t)((bytes + sizeof(apss_pair) - 1) / sizeof(apss_pair))));
+  }
+  ~HostBuf() { host_pairs_free(reinterpret_cast<apss_pair*>(p)); }
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  T& operator[](int64_t i) { return p[i]; }
+  const T& operator[](int64_t i) const { return p[i]; }
+  T* data() { return p; }
+};
+
 Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t self, int32_t kd_cap,
                   int32_t dm, bool all_lists) {
   Tiers tr;
@@ -349,7 +368,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
                        s.nnz, s.col, p, chunk, cnt);
     APSS_HIP(hipGetLastError());
   }
-  std::vector<int32_t> h((size_t)d + 1);
+  HostBuf<int32_t> h(d + 1);
   APSS_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int32_t) * (d + 1), hipMemcpyDeviceToHost, st));
   APSS_HIP(hipStreamSynchronize(st));
   ar.release(cnt);
@@ -382,13 +401,12 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
   // The bitmap index of a feature is its frequency rank, so every tile block
   // holds its segments from the most to the least frequent feature: threads
   // that own neighbouring entries see segments of similar length.
-  std::vector<int32_t> fm((size_t)d, F_IGNORE);
-  for (int64_t c = 0; c < std::min<int64_t>(p, d); ++c) fm[c] = F_HEAD;
-  for (int64_t c = p; c < d; ++c)
-    if (h[c] >= need) fm[c] = F_CURSOR;
+  HostBuf<int32_t> fm(d);
+  for (int64_t c = 0; c < d; ++c) fm[c] = c < p ? F_HEAD : h[c] >= need ? F_CURSOR : F_IGNORE;
   for (size_t k = 0; k < cand.size(); ++k) fm[cand[k]] = (int32_t)k;
   for (size_t k = 0; k < dense.size(); ++k) fm[dense[k]] = F_DENSE0 - (int32_t)k;
-  std::vector<int64_t> sp((size_t)d + 1, 0);
+  HostBuf<int64_t> sp(d + 1);
+  sp[0] = 0;
   std::vector<int32_t> small_lists;
   std::vector<SortChunk> chunks;  // lists of 65.. entries, cut in LDS-sized pieces
   int64_t max_long = 0;           // longest list cut in more than one piece
@@ -478,6 +496,7 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
     ar.release(lists);
     ar.release(cur);
   }
+  APSS_HIP(hipStreamSynchronize(st));  // the uploads from fm and sp are complete
   return tr;
 }
 
@@ -487,11 +506,10 @@ Tiers build_tiers(Arena& ar, hipStream_t st, const Side& s, int32_t p, int32_t s
 // Greedy cut of the rows into tiles of at most max_rows rows whose bitmap-tier
 // entries (padded to a multiple of 4) stay within cap_d and whose cursor-tier
 // entries stay within cap_s (a single row always forms a tile).
-static std::vector<int64_t> cut_tiles(const std::vector<int32_t>& dc, const std::vector<int32_t>& sc,
+static std::vector<int64_t> cut_tiles(const int32_t* dc, const int32_t* sc, int64_t n,
                                       int32_t max_rows, int32_t cap_d, int32_t cap_s,
                                       const std::vector<int64_t>& bounds) {
   std::vector<int64_t> ts;
-  const int64_t n = (int64_t)dc.size();
   ts.push_back(0);
   int64_t rows = 0, sd = 0, ss = 0;
   size_t nb = 0;  // next forced boundary (instance start)
@@ -522,14 +540,16 @@ TileSet build_tiles(Arena& ar, hipStream_t st, const Side& s, const Tiers& tr, i
     hipLaunchKernelGGL(k_row_counts, dim3(blocks_for(s.n, 4, 65536)), dim3(256), 0, st, s.n, s.rp,
                        s.col, tr.p, tr.fmap, cnts, cnts + s.n, cnts + 2 * s.n);
     APSS_HIP(hipGetLastError());
-    std::vector<int32_t> dc((size_t)s.n), sc((size_t)s.n), hc((size_t)s.n);
-    APSS_HIP(hipMemcpyAsync(dc.data(), cnts, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipMemcpyAsync(sc.data(), cnts + s.n, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
-    APSS_HIP(hipMemcpyAsync(hc.data(), cnts + 2 * s.n, sizeof(int32_t) * s.n, hipMemcpyDeviceToHost, st));
+    // the three count arrays in one pinned block: dc, sc, hc
+    HostBuf<int32_t> hcnt(3 * s.n);
+    APSS_HIP(hipMemcpyAsync(hcnt.data(), cnts, sizeof(int32_t) * 3 * s.n, hipMemcpyDeviceToHost, st));
     APSS_HIP(hipStreamSynchronize(st));
     ar.release(cnts);
-    if (!records) std::fill(sc.begin(), sc.end(), 0);
-    cuts = cut_tiles(dc, sc, max_rows, cap_d, cap_s, bounds);
+    int32_t* dc = hcnt.data();
+    int32_t* sc = dc + s.n;
+    const int32_t* hc = sc + s.n;
+    if (!records) std::fill(sc, sc + s.n, 0);
+    cuts = cut_tiles(dc, sc, s.n, max_rows, cap_d, cap_s, bounds);
     ts.rare.assign(cuts.size() - 1, 1);
     for (size_t k = 0; k + 1 < cuts.size(); ++k) {
       int64_t m = 0;
