@@ -4,7 +4,8 @@ heads), the large workload at t = 0.7 and, on a 1e5-row slice, at the split
 p = 4096 (the reference's prefix-norm filter and forward-list head dot), the
 256-instance batched workload with the autotuner on, the full small workload
 with the default tiering, a query whose result outgrows the first output
-buffer, and the device-built inverted index against the oracle's transpose.
+buffer, the device-built inverted index against the oracle's transpose, and the
+2e6-row serving corpus queried through the index and the server.
 Pairs touching a seeded sample of rows are compared with the FP64 oracle's
 answer for those rows against all rows (exact set; scores within 1e-6
 relative, or the stated BF16 bound); the whole sets are compared across forms
@@ -144,3 +145,37 @@ def test_transpose_long_lists_and_empty():
         assert np.array_equal(g, e)
     z = apss.transpose(np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32), 10, 0)
     assert np.array_equal(z[0], np.zeros(11, np.int64)) and len(z[1]) == 0
+
+
+def _rows(rp, col, val, a, b):
+    return ((rp[a:b + 1] - rp[a]).astype(np.int64), col[rp[a]:rp[b]].copy(), val[rp[a]:rp[b]].copy())
+
+
+def test_serving_workload_full_size():
+    # the serving configuration's 2e6-row corpus as a resident index: query
+    # rows from the serving query pool through apss_index_query (t = 0.6 and
+    # 0.8) and two requests packed by the server, against the FP64 oracle over
+    # the whole corpus
+    rp, col, val, d = apss.workload("serving")
+    qrp, qcol, qval, _ = apss.workload("serving", n=1 << 14, seed_offset=1000)
+    q1, q2 = _rows(qrp, qcol, qval, 100, 164), _rows(qrp, qcol, qval, 5000, 5040)
+    corpus = (rp, col, val)
+    exp = {t: oracle.query_fp64(q1, corpus, d, t) for t in (0.6, 0.8)}  # ~10 s each
+    ix = apss.Index(rp, col, val, d)
+    try:
+        for t in (0.6, 0.8):
+            got, info = ix.query(*q1, t)
+            print(f"serving corpus n={len(rp) - 1}: query t={t}: {len(got)} pairs", flush=True)
+            check_pairs(got, exp[t])
+        srv = apss.Server(ix, slots=2)
+        try:
+            tk1, tk2 = srv.submit(*q1, 0.6), srv.submit(*q2, 0.8)
+            g1, g2 = srv.wait(tk1)[0], srv.wait(tk2)[0]
+        finally:
+            srv.close()
+    finally:
+        ix.close()
+    check_pairs(g1, exp[0.6])
+    e2 = oracle.query_fp64(q2, corpus, d, 0.8)
+    check_pairs(g2, e2)
+    assert len(g1) > 0 and len(e2) > 0
